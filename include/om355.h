@@ -1,0 +1,128 @@
+/*
+ * om355.h -- C ABI of libom355.so, the MI355X (gfx950) depth-map kernels of py_ortho_amd.
+ *
+ * Plain pointers and sizes only.  Every device pointer is caller-owned device memory (a torch
+ * tensor's data_ptr() from Python); the library never frees caller memory.  `stream` is a
+ * hipStream_t (NULL = the default stream).  Every entry point returns 0 on success or a negative
+ * OM_E* code; om_last_error() then describes the failure (thread-local).  Calls only enqueue work
+ * on `stream` unless stated otherwise; they are reentrant across threads and streams.
+ *
+ * Reference interfaces replaced (pyOrthomosaic 0.9.3):
+ *   om_sweep_hypothesis   _dense.sweep_hypothesis          orthomosaic/_dense.pyx:740-838
+ *   om_sweep              the hypothesis loop of _sweep      orthomosaic/densify.py:336-342
+ *   om_sweep_finalize     the parabola of _sweep             orthomosaic/densify.py:354-361
+ *   om_pm_refine          _dense.pm_refine                  orthomosaic/_dense.pyx:566-666
+ *   om_box_zero_mean      _fast.box_zero_mean (densify._box) orthomosaic/_fast.pyx:661-713
+ *   om_hyp_linear         _fast.hyp_linear                  orthomosaic/_fast.pyx:628-641
+ *   om_hyp_refine         _fast.hyp_refine                  orthomosaic/_fast.pyx:644-658
+ *   om_half               densify._half                     orthomosaic/densify.py:127-130
+ *   om_ref_stats          mu / sd of _sweep                 orthomosaic/densify.py:327-328
+ */
+#ifndef OM355_H
+#define OM355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OM_OK = 0,
+    OM_EINVAL = -1,   /* bad argument (reference: ValueError) */
+    OM_ESHAPE = -2,   /* shapes disagree (reference: ValueError) */
+    OM_ENOMEM = -3,   /* allocation failed (reference: MemoryError) */
+    OM_EHIP = -4,     /* HIP runtime error */
+    OM_ENSRC = -5,    /* number of sources outside 1..16 (reference: ValueError) */
+    OM_EUNSUP = -6    /* valid input the library does not handle */
+};
+
+/* hypothesis generators of om_sweep (inverse depth of hypothesis i at pixel p) */
+enum {
+    OM_HYP_MAPS = 0,   /* hyp_a: (nh, H, W) float32 maps, map j is hypothesis i0 + j */
+    OM_HYP_LINEAR = 1, /* hyp_a = base, hyp_b = step: base + f32(i) * step (densify.py:411-413) */
+    OM_HYP_REFINE = 2, /* hyp_a = iu, hyp_b = ds: max(iu + f32(i - hyp_offset) * ds, 1e-6) (densify.py:423-426) */
+    OM_HYP_CONST = 3   /* hyp_a: (nh,) float32 table, constant over the image (fronto-parallel planes) */
+};
+
+/* One source view.  img: (h, w) float32 device image, NaN = masked.  geo: the reference's 18
+ * values per source (_dense.pyx:748): Rs row-major (9), b = Rs (C_ref - C_src) (3), f, k1, k2, k3,
+ * cx, cy. */
+typedef struct om_src {
+    const float *img;
+    int32_t h, w;
+    double geo[18];
+} om_src;
+
+typedef struct om_sweep_desc {
+    const float *ref, *mu, *sd; /* (H, W) float32: reference image (masked = 0) and its window mean/std */
+    const float *ray;           /* (3, H, W) float32 world rays, camera-axis depth 1 */
+    int32_t H, W;
+    int32_t n_src;              /* 1..16 */
+    const om_src *src;          /* host array of n_src descriptors */
+    int32_t r, top_k;           /* window radius (window 2r+1), score = mean of the best top_k sources */
+    int32_t hyp_mode;           /* OM_HYP_* */
+    const float *hyp_a, *hyp_b; /* generator inputs (device) */
+    int32_t hyp_offset;         /* OM_HYP_REFINE: t = i - hyp_offset */
+    int32_t i0, nh;             /* hypotheses i0 .. i0 + nh - 1 (absolute indices for the winner) */
+    /* streaming winner state (H, W), updated in place: _sweep initialises best/prev/spb/snb to -2
+     * and idx to -1 (densify.py:331-335) */
+    float *best, *prev, *s_prev_best, *s_next_best;
+    int32_t *idx;
+    /* device workspace for hypothesis chunking; may be NULL (then one chunk) */
+    void *workspace;
+    size_t workspace_bytes;
+    int32_t chunks;             /* 0 = choose from the image size; 1 = no chunking */
+    int32_t tile_h;             /* 0 = default tile height; else a multiple of 4 */
+} om_sweep_desc;
+
+/* library */
+int om_version(void);
+const char *om_last_error(void);
+/* name (>= 64 bytes), compute units, clock kHz, global memory bytes of the current device */
+int om_device_info(char *name, int32_t name_len, int32_t *cus, int32_t *clock_khz, int64_t *mem_bytes);
+
+/* fused plane sweep: hypotheses [i0, i0+nh) of one view in one launch (+ a merge when chunked) */
+size_t om_sweep_workspace_bytes(const om_sweep_desc *d);
+int om_sweep(const om_sweep_desc *d, void *stream);
+
+/* _dense.sweep_hypothesis, one hypothesis i with an explicit (H, W) inverse-depth map `inv`:
+ * imgs/img_hw: n device image pointers and their (h, w); geo: host (n, 18) float64. */
+int om_sweep_hypothesis(const float *ref, const float *mu, const float *sd, int32_t H, int32_t W, int32_t n,
+                        const float *const *imgs, const int32_t *img_hw, const float *ray, const double *geo,
+                        const float *inv, int32_t r, int32_t top_k, int32_t i, float *best, float *prev,
+                        float *s_prev_best, float *s_next_best, int32_t *idx, void *stream);
+
+/* parabola of _sweep (densify.py:354-361): depth = 1 / max(inv0 + (idx + off) * dinv, 1e-9) as
+ * float32, with inv0/dinv the first hypothesis map and the spacing of d's generator (n = d->nh
+ * hypotheses starting at d->i0 = 0). */
+int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *stream);
+
+/* PatchMatch repair of one view: _dense.pm_refine.  Device arrays: ref (H,W) NaN = masked,
+ * ray (3,H,W), depth (H,W), normal (3,H,W), score (H,W) in/out, active (H,W) uint8; imgs/deps:
+ * S device pointers (deps[s] may be NULL), ih/iw/par (S,18)/Cr(3): host.  *changed receives the
+ * number of accepted hypothesis changes (the reference's return value); it synchronises `stream`
+ * when changed != NULL.  workspace: om_pm_workspace_bytes(H, W) bytes of device memory. */
+size_t om_pm_workspace_bytes(int32_t H, int32_t W);
+int om_pm_refine(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
+                 const float *const *imgs, const float *const *deps, const int32_t *ih, const int32_t *iw,
+                 const double *par, float *depth, float *normal, float *score, const uint8_t *active, int32_t r,
+                 int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters, double dmin, double dmax,
+                 uint32_t seed, void *workspace, size_t workspace_bytes, int64_t *changed, void *stream);
+
+/* depth helpers */
+int om_hyp_linear(const float *base, const float *step, int64_t n, int64_t i, float *out, void *stream);
+int om_hyp_refine(const float *iu, const float *ds, int64_t n, int64_t t, float *out, void *stream);
+int om_box_zero_mean(const float *a, int32_t H, int32_t W, int32_t r, float *out, void *workspace,
+                     size_t workspace_bytes, void *stream);
+size_t om_box_workspace_bytes(int32_t H, int32_t W, int32_t r);
+/* mu = box(ref), sd = sqrt(max(box(ref*ref) - mu*mu, 1e-6)) (densify.py:327-328) */
+int om_ref_stats(const float *ref, int32_t H, int32_t W, int32_t r, float *mu, float *sd, void *workspace,
+                 size_t workspace_bytes, void *stream);
+int om_half(const float *g, int32_t H, int32_t W, float *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+"""ctypes binding of libom355.so (include/om355.h).
+
+The shared library is the only compute path: if it is missing or cannot be loaded, every entry
+point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libom355.so")
+
+OM_OK, OM_EINVAL, OM_ESHAPE, OM_ENOMEM, OM_EHIP, OM_ENSRC, OM_EUNSUP = 0, -1, -2, -3, -4, -5, -6
+OM_HYP_MAPS, OM_HYP_LINEAR, OM_HYP_REFINE, OM_HYP_CONST = 0, 1, 2, 3
+
+c_void_p = ctypes.c_void_p
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_size_t = ctypes.c_size_t
+c_double = ctypes.c_double
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class OmSrc(ctypes.Structure):
+    _fields_ = [("img", c_void_p), ("h", c_int32), ("w", c_int32), ("geo", c_double * 18)]
+
+
+class OmSweepDesc(ctypes.Structure):
+    _fields_ = [
+        ("ref", c_void_p), ("mu", c_void_p), ("sd", c_void_p), ("ray", c_void_p),
+        ("H", c_int32), ("W", c_int32),
+        ("n_src", c_int32), ("src", ctypes.POINTER(OmSrc)),
+        ("r", c_int32), ("top_k", c_int32),
+        ("hyp_mode", c_int32), ("hyp_a", c_void_p), ("hyp_b", c_void_p), ("hyp_offset", c_int32),
+        ("i0", c_int32), ("nh", c_int32),
+        ("best", c_void_p), ("prev", c_void_p), ("s_prev_best", c_void_p), ("s_next_best", c_void_p),
+        ("idx", c_void_p),
+        ("workspace", c_void_p), ("workspace_bytes", c_size_t),
+        ("chunks", c_int32), ("tile_h", c_int32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/om355.h
+SIGNATURES = {
+    "om_version": (c_int32, []),
+    "om_last_error": (ctypes.c_char_p, []),
+    "om_device_info": (c_int32, [ctypes.c_char_p, c_int32, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+                                 ctypes.POINTER(c_int64)]),
+    "om_sweep_workspace_bytes": (c_size_t, [ctypes.POINTER(OmSweepDesc)]),
+    "om_sweep": (c_int32, [ctypes.POINTER(OmSweepDesc), c_void_p]),
+    "om_sweep_hypothesis": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                      ctypes.POINTER(c_void_p), ctypes.POINTER(c_int32), c_void_p,
+                                      ctypes.POINTER(c_double), c_void_p, c_int32, c_int32, c_int32,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "om_sweep_finalize": (c_int32, [ctypes.POINTER(OmSweepDesc), c_void_p, c_void_p]),
+    "om_pm_workspace_bytes": (c_size_t, [c_int32, c_int32]),
+    "om_pm_refine": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, ctypes.POINTER(c_double), c_int32,
+                               ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_int32),
+                               ctypes.POINTER(c_int32), ctypes.POINTER(c_double), c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_int32, c_int32, c_int32, c_double, c_double, c_int32, c_double,
+                               c_double, ctypes.c_uint32, c_void_p, c_size_t, ctypes.POINTER(c_int64), c_void_p]),
+    "om_hyp_linear": (c_int32, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "om_hyp_refine": (c_int32, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "om_box_workspace_bytes": (c_size_t, [c_int32, c_int32, c_int32]),
+    "om_box_zero_mean": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "om_ref_stats": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_size_t,
+                               c_void_p]),
+    "om_half": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class NativeLibraryMissing(ImportError):
+    pass
+
+
+def lib():
+    """The loaded libom355.so (raises NativeLibraryMissing when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise NativeLibraryMissing(
+                    f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                    "(hipcc --offload-arch=gfx950)")
+            h = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = h
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().om_last_error()
+    return msg.decode("utf-8", "replace") if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    """Map an OM_E* code to the exception type the reference raises for the same condition."""
+    if rc == OM_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if rc in (OM_EINVAL, OM_ESHAPE, OM_ENSRC):
+        raise ValueError(msg)
+    if rc == OM_ENOMEM:
+        raise MemoryError(msg)
+    if rc == OM_EUNSUP:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)

@@ -1,0 +1,41 @@
+// Shared host/device helpers of libom355 (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/om355.h"
+
+namespace om {
+
+// thread-local message behind om_last_error()
+void set_error(const char *fmt, ...);
+
+inline int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    set_error("%s", buf);
+    return code;
+}
+
+inline int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess)
+        return OM_OK;
+    return fail(e == hipErrorOutOfMemory ? OM_ENOMEM : OM_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// launch check: configuration errors surface at launch time
+inline int launch_check(const char *what) { return hip_check(hipGetLastError(), what); }
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace om

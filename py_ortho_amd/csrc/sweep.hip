@@ -1,0 +1,627 @@
+// Fused plane sweep for gfx950: every hypothesis of one reference view in one launch.
+//
+// Reference semantics: _dense.sweep_hypothesis (orthomosaic/_dense.pyx:740-838) with the row
+// kernels of orthomosaic/_sweep.h:33-214, looped over hypotheses by densify._sweep
+// (orthomosaic/densify.py:336-342).
+//
+// Work decomposition.  A workgroup (256 threads, 4 wavefronts) owns a 64-column x TH-row tile of
+// output pixels and a contiguous chunk of hypotheses.  For each (hypothesis, source) it
+//   W: warps the tile plus an r-pixel halo into LDS (FP64 projection + FP64 bilinear, the
+//      reference's exact operation sequence; NaN marks an invalid sample),
+//   H: forms the horizontal window sums of V, J, J^2, J*I for every halo row (FP64 running sums
+//      over 8-column runs, stored as float32 exactly where the reference rounds them),
+//   V: forms the vertical sums in FP64 and the NCC of each output pixel, and folds it into a
+//      per-pixel top-k kept in registers.
+// After the last source of a hypothesis the per-pixel winner state (best, prev, s_prev_best,
+// s_next_best, idx) is updated in registers; it is read once and written once per launch.
+//
+// Exactness.  The library is compiled with -ffp-contract=off and uses IEEE division and square
+// root, so every sample, window sum and score is computed with the reference's operations in the
+// reference's order.  The reference forms window sums as running sums along the whole row
+// (_sweep.h:96-113) and down the whole column (_sweep.h:139-146); the kernel forms them over an
+// 8-sample run and a TH/4-row run.  Both equal the exact sum whenever no partial sum rounds
+// (float32 inputs, FP64 accumulators: any window of values whose magnitudes span less than
+// 2^29), so the float32 outputs are bit-identical to the reference in practice and within an
+// FP64 rounding of it otherwise.
+//
+// Hypothesis chunking.  Small images give too few tiles to fill 256 CUs, so the hypotheses are
+// split into chunks processed by different workgroups.  Each chunk records its local winner and
+// the scores at its ends; om_sweep_merge replays the streaming winner rule over the chunks in
+// order, which reproduces the sequential result exactly.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "om_common.h"
+
+namespace om {
+
+constexpr int kTW = 64;                           // output columns per tile
+constexpr int kThreads = 256;                     // threads per workgroup
+constexpr int kRun = 8;                           // columns per thread in the horizontal pass
+constexpr int kRunsPerRow = kTW / kRun;           // 8
+constexpr int kHRows = kThreads / kRunsPerRow;    // 32 halo rows per horizontal pass
+constexpr int kVSeg = kThreads / kTW;             // 4 row segments per column in the vertical pass
+constexpr int kMaxR = 7;                          // largest window radius of the fused kernel
+constexpr int kMaxSlots = 10;                     // halo pixels cached per thread: 10*256 >= 32*78
+constexpr int kPartPlanes = 6;                    // chunk partials: best, idx, prev-of-best, next-of-best, first, last
+
+struct SrcK {
+    const float *img;
+    int h, w;
+    double g[18];
+};
+
+struct SweepK {
+    const float *ref, *mu, *sd, *ray;
+    int H, W, n, r, k;        // k = min(top_k, n)
+    int mode;                 // OM_HYP_*
+    const float *ha, *hb;
+    int hoff;
+    int i0, nh, hc, chunks;   // hypotheses [i0, i0+nh), hc per chunk
+    int TH, tiles_x, tiles_y;
+    int pinhole;              // every source has k1 = k2 = k3 = 0
+    float *best, *prev, *spb, *snb;
+    int *idx;
+    float *part;              // (chunks, 6, H, W) when chunks > 1
+    SrcK src[16];
+};
+
+__device__ __forceinline__ float hyp_value(const SweepK &P, int i, int p, float a, float b)
+{
+    // a, b: the generator's per-pixel inputs at p (base/step or iu/ds)
+    switch (P.mode) {
+    case OM_HYP_LINEAR: {
+        const float t = (float)i * b;  // _fast.pyx:638-640, no contraction
+        return a + t;
+    }
+    case OM_HYP_REFINE: {
+        float v = (float)(i - P.hoff) * b;  // _fast.pyx:654-658
+        v = a + v;
+        return (v >= 1e-6f || v != v) ? v : 1e-6f;
+    }
+    case OM_HYP_CONST:
+        return P.ha[i - P.i0];
+    default:
+        return P.ha[(size_t)(i - P.i0) * ((size_t)P.H * P.W) + p];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void topk_insert(float (&t)[K], float v)
+{
+    // keep t sorted descending; equal values keep their order (irrelevant for the sum)
+#pragma unroll
+    for (int a = K - 1; a >= 0; --a) {
+        const bool lt = t[a] < v;
+        const float up = (a > 0) ? t[a - 1] : v;
+        const bool prev_lt = (a > 0) ? (t[a - 1] < v) : true;
+        if (lt)
+            t[a] = prev_lt ? up : v;
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
+{
+    extern __shared__ float lds[];
+    const int r = P.r, TH = P.TH;
+    const int HR = TH + 2 * r;          // halo rows
+    const int HC = kTW + 2 * r;         // halo columns
+    const int NHALO = HR * HC;
+    float *sI = lds;                    // (HR, HC) reference
+    float *sJ = sI + NHALO;             // (HR, HC) warped source, NaN = invalid
+    float *sH = sJ + NHALO;             // (4, HR, kTW) horizontal sums
+
+    const int tid = threadIdx.x;
+    const int ntiles = P.tiles_x * P.tiles_y;
+    const int tile = blockIdx.x % ntiles;
+    const int chunk = blockIdx.x / ntiles;
+    const int tx0 = (tile % P.tiles_x) * kTW;
+    const int ty0 = (tile / P.tiles_x) * TH;
+    const int ia = P.i0 + chunk * P.hc;
+    const int ib = min(P.i0 + P.nh, ia + P.hc);
+    const int H = P.H, W = P.W;
+    const size_t HW = (size_t)H * W;
+
+    // ---- halo pixels owned by this thread in the warp pass: ray and generator inputs cached
+    bool hin[kMaxSlots];
+    int hpix[kMaxSlots];
+    float hrx[kMaxSlots], hry[kMaxSlots], hrz[kMaxSlots], hga[kMaxSlots], hgb[kMaxSlots];
+    double hd[kMaxSlots];
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; ++s) {
+        const int hp = tid + s * kThreads;
+        const int hy = hp / HC, hx = hp - hy * HC;
+        const int gy = ty0 - r + hy, gx = tx0 - r + hx;
+        const bool in = hp < NHALO && gy >= 0 && gy < H && gx >= 0 && gx < W;
+        hin[s] = in;
+        const int p = in ? gy * W + gx : 0;
+        hpix[s] = p;
+        hrx[s] = in ? P.ray[p] : 0.f;
+        hry[s] = in ? P.ray[HW + p] : 0.f;
+        hrz[s] = in ? P.ray[2 * HW + p] : 0.f;
+        hga[s] = (in && (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE)) ? P.ha[p] : 0.f;
+        hgb[s] = (in && (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE)) ? P.hb[p] : 0.f;
+        if (hp < NHALO)
+            sI[hp] = in ? P.ref[p] : 0.f;
+        hd[s] = 0.0;
+    }
+
+    // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+VR-1
+    constexpr int kMaxVR = 8;  // TH <= 32 -> VR <= 8
+    const int VR = TH / kVSeg;
+    const int vc = tid % kTW, vseg = tid / kTW, vy0 = vseg * VR;
+    const int ox = tx0 + vc;
+    float mu_[kMaxVR], sd_[kMaxVR];
+    float st_best[kMaxVR], st_prev[kMaxVR], st_spb[kMaxVR], st_snb[kMaxVR], st_first[kMaxVR];
+    int st_idx[kMaxVR];
+    float tk[kMaxVR][K];
+    const bool chunked = P.chunks > 1;
+#pragma unroll
+    for (int j = 0; j < kMaxVR; ++j) {
+        const int oy = ty0 + vy0 + j;
+        const bool in = j < VR && oy < H && ox < W;
+        const size_t p = in ? (size_t)oy * W + ox : 0;
+        mu_[j] = in ? P.mu[p] : 0.f;
+        sd_[j] = in ? P.sd[p] : 1.f;
+        if (chunked) {
+            st_best[j] = -INFINITY;
+            st_prev[j] = NAN;
+            st_spb[j] = NAN;
+            st_snb[j] = NAN;
+            st_idx[j] = INT_MIN;
+        } else {
+            st_best[j] = in ? P.best[p] : 0.f;
+            st_prev[j] = in ? P.prev[p] : 0.f;
+            st_spb[j] = in ? P.spb[p] : 0.f;
+            st_snb[j] = in ? P.snb[p] : 0.f;
+            st_idx[j] = in ? P.idx[p] : 0;
+        }
+        st_first[j] = NAN;
+    }
+    const double kk = 1.0 / (double)((2 * r + 1) * (2 * r + 1));
+
+    for (int i = ia; i < ib; ++i) {
+        // inverse depth -> depth per cached halo pixel (shared by all sources)
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; ++s) {
+            if (hin[s]) {
+                const float inv = hyp_value(P, i, hpix[s], hga[s], hgb[s]);
+                hd[s] = 1.0 / (double)inv;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxVR; ++j)
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+                tk[j][a] = -INFINITY;
+
+        for (int s = 0; s < P.n; ++s) {
+            const SrcK &S = P.src[s];
+            const double *g = S.g;
+            const double wl = (double)S.w - 1.001, hl = (double)S.h - 1.001;
+            const float *img = S.img;
+            const int sw = S.w;
+            // ---- W: warp the halo into LDS (_sweep.h:49-80)
+#pragma unroll
+            for (int q = 0; q < kMaxSlots; ++q) {
+                const int hp = tid + q * kThreads;
+                if (hp < NHALO) {
+                    float J = NAN;
+                    if (hin[q]) {
+                        const double d = hd[q];
+                        const double rx = hrx[q], ry = hry[q], rz = hrz[q];
+                        const double X = g[9] + d * ((g[0] * rx + g[1] * ry) + g[2] * rz);
+                        const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
+                        const double Z = g[11] + d * ((g[6] * rx + g[7] * ry) + g[8] * rz);
+                        if (!(Z <= 1e-6)) {
+                            const double nx = X / Z, ny = Y / Z;
+                            double dd;
+                            if (P.pinhole) {
+                                dd = g[12];  // f * (1 + r2 * 0) == f for finite r2
+                            } else {
+                                const double rr = nx * nx + ny * ny;
+                                dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
+                            }
+                            const double u = dd * nx + g[16], v = dd * ny + g[17];
+                            if (u >= 0.0 && v >= 0.0 && u <= wl && v <= hl) {
+                                const int x0 = (int)u, y0 = (int)v;
+                                const double fx = u - (double)x0, fy = v - (double)y0;
+                                const float *pp = img + (size_t)y0 * sw + x0;
+                                const double a00 = pp[0], a01 = pp[1], a10 = pp[sw], a11 = pp[sw + 1];
+                                const double val =
+                                    (a00 * (1.0 - fx) + a01 * fx) * (1.0 - fy) + (a10 * (1.0 - fx) + a11 * fx) * fy;
+                                if (isfinite(val))
+                                    J = (float)val;
+                            }
+                        }
+                    }
+                    sJ[hp] = J;
+                }
+            }
+            __syncthreads();
+            // ---- H: horizontal window sums of one 8-column run per thread (_sweep.h:85-114)
+            for (int hr = tid / kRunsPerRow; hr < HR; hr += kHRows) {
+                const float *Jr = sJ + hr * HC, *Ir = sI + hr * HC;
+                const int c0 = (tid % kRunsPerRow) * kRun;
+                int sv = 0;
+                double sj = 0.0, sjj = 0.0, sij = 0.0;
+                for (int t = 0; t <= 2 * r; ++t) {
+                    const float J = Jr[c0 + t];
+                    const bool ok = J == J;
+                    const float jv = ok ? J : 0.f;
+                    sv += ok;
+                    sj = sj + (double)jv;
+                    sjj = sjj + (double)(jv * jv);
+                    sij = sij + (double)(jv * Ir[c0 + t]);
+                }
+                float *h0 = sH + hr * kTW, *h1 = h0 + HR * kTW, *h2 = h1 + HR * kTW, *h3 = h2 + HR * kTW;
+                h0[c0] = (float)sv;
+                h1[c0] = (float)sj;
+                h2[c0] = (float)sjj;
+                h3[c0] = (float)sij;
+#pragma unroll
+                for (int c = c0 + 1; c < c0 + kRun; ++c) {
+                    const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
+                    const bool okn = Jn == Jn, oko = Jo == Jo;
+                    const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
+                    sv += (int)okn - (int)oko;
+                    sj = sj + (double)jn;
+                    sjj = sjj + (double)(jn * jn);
+                    sij = sij + (double)(jn * Ir[c + 2 * r]);
+                    sj = sj - (double)jo;
+                    sjj = sjj - (double)(jo * jo);
+                    sij = sij - (double)(jo * Ir[c - 1]);
+                    h0[c] = (float)sv;
+                    h1[c] = (float)sj;
+                    h2[c] = (float)sjj;
+                    h3[c] = (float)sij;
+                }
+            }
+            __syncthreads();
+            // ---- V: vertical sums and NCC (_sweep.h:116-178)
+            {
+                const float *h0 = sH + vc, *h1 = h0 + HR * kTW, *h2 = h1 + HR * kTW, *h3 = h2 + HR * kTW;
+                double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+                for (int t = 0; t <= 2 * r; ++t) {
+                    const int o = (vy0 + t) * kTW;
+                    c0 = c0 + (double)h0[o];
+                    c1 = c1 + (double)h1[o];
+                    c2 = c2 + (double)h2[o];
+                    c3 = c3 + (double)h3[o];
+                }
+#pragma unroll
+                for (int j = 0; j < kMaxVR; ++j) {
+                    if (j < VR) {
+                        if (j > 0) {
+                            const int on = (vy0 + j + 2 * r) * kTW, oo = (vy0 + j - 1) * kTW;
+                            double t;
+                            t = c0 + (double)h0[on];
+                            c0 = t - (double)h0[oo];
+                            t = c1 + (double)h1[on];
+                            c1 = t - (double)h1[oo];
+                            t = c2 + (double)h2[on];
+                            c2 = t - (double)h2[oo];
+                            t = c3 + (double)h3[on];
+                            c3 = t - (double)h3[oo];
+                        }
+                        float out;
+                        const double cov = c0 * kk;
+                        if (cov <= 0.9) {
+                            out = -1.0f;
+                        } else {
+                            const double mj = c1 * kk / cov;
+                            const double m2 = c2 * kk / cov;
+                            const double mij = c3 * kk / cov;
+                            double sj = m2 - mj * mj;
+                            sj = sj > 1e-6 ? sqrt(sj) : 1e-3;
+                            const double q = (mij - (double)mu_[j] * mj) / ((double)sd_[j] * sj);
+                            out = q > 1.0 ? 1.0f : (q < -1.0 ? -1.0f : (float)q);
+                        }
+                        topk_insert<K>(tk[j], out);
+                    }
+                }
+            }
+            // the next source's W pass writes sJ only (last read before the barrier above) and
+            // its H pass writes sH after the next barrier, so no barrier is needed here
+        }
+        // ---- combine + streaming winner (_sweep.h:181-214)
+#pragma unroll
+        for (int j = 0; j < kMaxVR; ++j) {
+            if (j < VR) {
+                float s = 0.0f;
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+                    s = s + tk[j][a];
+                s = s / (float)K;
+                if (i == ia)
+                    st_first[j] = s;
+                if (st_idx[j] == i - 1)
+                    st_snb[j] = s;
+                if (s > st_best[j]) {
+                    st_best[j] = s;
+                    st_spb[j] = st_prev[j];
+                    st_snb[j] = chunked ? NAN : -2.0f;
+                    st_idx[j] = i;
+                }
+                st_prev[j] = s;
+            }
+        }
+    }
+
+#pragma unroll
+    for (int j = 0; j < kMaxVR; ++j) {
+        const int oy = ty0 + vy0 + j;
+        if (j < VR && oy < H && ox < W) {
+            const size_t p = (size_t)oy * W + ox;
+            if (chunked) {
+                float *pc = P.part + (size_t)chunk * kPartPlanes * HW;
+                pc[p] = st_best[j];
+                pc[HW + p] = __int_as_float(st_idx[j]);
+                pc[2 * HW + p] = st_spb[j];
+                pc[3 * HW + p] = st_snb[j];
+                pc[4 * HW + p] = st_first[j];
+                pc[5 * HW + p] = st_prev[j];
+            } else {
+                P.best[p] = st_best[j];
+                P.prev[p] = st_prev[j];
+                P.spb[p] = st_spb[j];
+                P.snb[p] = st_snb[j];
+                P.idx[p] = st_idx[j];
+            }
+        }
+    }
+}
+
+// Replays the streaming winner rule (_sweep.h:204-212) over the chunks in hypothesis order.
+__global__ void sweep_merge_kernel(const float *__restrict__ part, int chunks, int64_t HW, float *best, float *prev,
+                                   float *spb, float *snb, int *idx, int i0, int hc)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= HW)
+        return;
+    float gb = best[p], gp = prev[p], gs = spb[p], gn = snb[p];
+    int gi = idx[p];
+    for (int c = 0; c < chunks; ++c) {
+        const float *pc = part + (size_t)c * kPartPlanes * HW;
+        const float cb = pc[p], cp = pc[2 * HW + p], cn = pc[3 * HW + p], cf = pc[4 * HW + p], cl = pc[5 * HW + p];
+        const int ci = __float_as_int(pc[HW + p]);
+        const int a = i0 + c * hc;
+        if (gi == a - 1)
+            gn = cf;
+        if (cb > gb) {
+            gb = cb;
+            gi = ci;
+            gs = (cp != cp) ? gp : cp;
+            if (cn != cn)
+                gn = (c + 1 < chunks) ? part[(size_t)(c + 1) * kPartPlanes * HW + 4 * HW + p] : -2.0f;
+            else
+                gn = cn;
+        }
+        gp = cl;
+    }
+    best[p] = gb;
+    prev[p] = gp;
+    spb[p] = gs;
+    snb[p] = gn;
+    idx[p] = gi;
+}
+
+// parabola of densify._sweep (densify.py:354-361), NumPy's float32 / float64 promotion kept
+__global__ void sweep_finalize_kernel(SweepK P, float *depth)
+{
+    const int64_t HW = (int64_t)P.H * P.W;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= HW)
+        return;
+    const float a = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.ha[p] : 0.f;
+    const float b = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.hb[p] : 0.f;
+    const float inv0 = hyp_value(P, 0, (int)p, a, b);
+    const int n = P.nh;
+    const float dinv = n > 1 ? hyp_value(P, 1, (int)p, a, b) - inv0 : 0.0f;
+    const float best = P.best[p], spb = P.spb[p], snb = P.snb[p];
+    const int idx = P.idx[p];
+    const float den = (spb - 2.0f * best) + snb;
+    const bool interior = idx > 0 && idx < n - 1 && den < -1e-6f;
+    float off = interior ? (0.5f * (spb - snb)) / den : 0.0f;
+    off = off < -0.5f ? -0.5f : (off > 0.5f ? 0.5f : off);
+    const double inv = (double)inv0 + ((double)idx + (double)off) * (double)dinv;
+    const double m = (inv > 1e-9 || inv != inv) ? inv : 1e-9;
+    depth[p] = (float)(1.0 / m);
+}
+
+static int tile_height(int r) { return r <= kMaxR ? ((kHRows - 2 * r) / kVSeg) * kVSeg : 0; }
+
+static int build_params(const om_sweep_desc *d, SweepK &P)
+{
+    if (!d)
+        return fail(OM_EINVAL, "om_sweep: null descriptor");
+    if (d->n_src < 1 || d->n_src > 16)
+        return fail(OM_ENSRC, "sweep needs 1..16 sources (got %d)", d->n_src);
+    if (d->r < 0)
+        return fail(OM_EINVAL, "negative window radius");
+    if (d->H < 0 || d->W < 0 || d->nh < 0)
+        return fail(OM_ESHAPE, "negative size");
+    if (d->r > kMaxR)
+        return fail(OM_EUNSUP, "fused sweep supports window radius <= %d (got %d)", kMaxR, d->r);
+    if (d->top_k < 1)
+        return fail(OM_EINVAL, "top_k must be >= 1");
+    const int k = d->top_k < d->n_src ? d->top_k : d->n_src;
+    if (k > 4)
+        return fail(OM_EUNSUP, "fused sweep supports min(top_k, n) <= 4 (got %d)", k);
+    if (d->hyp_mode < OM_HYP_MAPS || d->hyp_mode > OM_HYP_CONST)
+        return fail(OM_EINVAL, "unknown hypothesis generator %d", d->hyp_mode);
+    if (!d->src)
+        return fail(OM_EINVAL, "null source list");
+    memset(&P, 0, sizeof P);
+    P.ref = d->ref;
+    P.mu = d->mu;
+    P.sd = d->sd;
+    P.ray = d->ray;
+    P.H = d->H;
+    P.W = d->W;
+    P.n = d->n_src;
+    P.r = d->r;
+    P.k = k;
+    P.mode = d->hyp_mode;
+    P.ha = d->hyp_a;
+    P.hb = d->hyp_b;
+    P.hoff = d->hyp_offset;
+    P.i0 = d->i0;
+    P.nh = d->nh;
+    P.best = d->best;
+    P.prev = d->prev;
+    P.spb = d->s_prev_best;
+    P.snb = d->s_next_best;
+    P.idx = d->idx;
+    P.pinhole = 1;
+    for (int s = 0; s < d->n_src; ++s) {
+        const om_src &o = d->src[s];
+        P.src[s].img = o.img;
+        P.src[s].h = o.h;
+        P.src[s].w = o.w;
+        for (int t = 0; t < 18; ++t)
+            P.src[s].g[t] = o.geo[t];
+        if (o.geo[13] != 0.0 || o.geo[14] != 0.0 || o.geo[15] != 0.0)
+            P.pinhole = 0;
+    }
+    P.TH = d->tile_h > 0 ? d->tile_h : tile_height(d->r);
+    if (P.TH < kVSeg || P.TH % kVSeg || P.TH + 2 * d->r > kHRows || P.TH / kVSeg > 8)
+        return fail(OM_EINVAL, "tile height %d invalid for r=%d", P.TH, d->r);
+    P.tiles_x = (int)cdiv(d->W, kTW);
+    P.tiles_y = (int)cdiv(d->H, P.TH);
+    return OM_OK;
+}
+
+static int choose_chunks(const om_sweep_desc *d, const SweepK &P)
+{
+    const int64_t tiles = (int64_t)P.tiles_x * P.tiles_y;
+    int c = d->chunks;
+    if (c <= 0) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        const int64_t want = 4LL * cus;  // ~ resident workgroups of the chip, several rounds
+        c = (int)std::min<int64_t>(std::max<int64_t>(cdiv(want, std::max<int64_t>(tiles, 1)), 1), 64);
+    }
+    c = std::max(1, std::min(c, std::max(d->nh, 1)));
+    return c;
+}
+
+}  // namespace om
+
+using namespace om;
+
+extern "C" size_t om_sweep_workspace_bytes(const om_sweep_desc *d)
+{
+    SweepK P;
+    if (build_params(d, P) != OM_OK)
+        return 0;
+    const int c = choose_chunks(d, P);
+    if (c <= 1)
+        return 0;
+    return (size_t)c * kPartPlanes * (size_t)d->H * d->W * sizeof(float);
+}
+
+extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
+{
+    SweepK Pl;
+    int rc = build_params(d, Pl);
+    if (rc)
+        return rc;
+    if (Pl.H == 0 || Pl.W == 0 || Pl.nh == 0)
+        return OM_OK;
+    int chunks = choose_chunks(d, Pl);
+    if (chunks > 1) {
+        const size_t need = (size_t)chunks * kPartPlanes * (size_t)Pl.H * Pl.W * sizeof(float);
+        if (!d->workspace || d->workspace_bytes < need)
+            chunks = 1;  // no room for partials: one chunk
+    }
+    Pl.hc = (int)cdiv(Pl.nh, chunks);
+    Pl.chunks = (int)cdiv(Pl.nh, Pl.hc);
+    Pl.part = (float *)d->workspace;
+    const int r = Pl.r;
+    const size_t halo = (size_t)(Pl.TH + 2 * r) * (kTW + 2 * r);
+    const size_t lds = sizeof(float) * (2 * halo + 4 * (size_t)(Pl.TH + 2 * r) * kTW);
+    const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
+    hipStream_t st = as_stream(stream);
+    switch (Pl.k) {
+    case 1: hipLaunchKernelGGL(sweep_kernel<1>, grid, dim3(kThreads), lds, st, Pl); break;
+    case 2: hipLaunchKernelGGL(sweep_kernel<2>, grid, dim3(kThreads), lds, st, Pl); break;
+    case 3: hipLaunchKernelGGL(sweep_kernel<3>, grid, dim3(kThreads), lds, st, Pl); break;
+    default: hipLaunchKernelGGL(sweep_kernel<4>, grid, dim3(kThreads), lds, st, Pl); break;
+    }
+    if ((rc = launch_check("sweep_kernel")))
+        return rc;
+    if (Pl.chunks > 1) {
+        const int64_t HW = (int64_t)Pl.H * Pl.W;
+        hipLaunchKernelGGL(sweep_merge_kernel, dim3((unsigned)cdiv(HW, 256)), dim3(256), 0, st, Pl.part, Pl.chunks,
+                           HW, Pl.best, Pl.prev, Pl.spb, Pl.snb, Pl.idx, Pl.i0, Pl.hc);
+        if ((rc = launch_check("sweep_merge_kernel")))
+            return rc;
+    }
+    return OM_OK;
+}
+
+extern "C" int om_sweep_hypothesis(const float *ref, const float *mu, const float *sd, int32_t H, int32_t W,
+                                   int32_t n, const float *const *imgs, const int32_t *img_hw, const float *ray,
+                                   const double *geo, const float *inv, int32_t r, int32_t top_k, int32_t i,
+                                   float *best, float *prev, float *s_prev_best, float *s_next_best, int32_t *idx,
+                                   void *stream)
+{
+    if (n < 1 || n > 16)
+        return fail(OM_ENSRC, "sweep_hypothesis needs 1..16 sources");
+    if (!imgs || !img_hw || !geo)
+        return fail(OM_EINVAL, "null source arrays");
+    om_src src[16];
+    for (int s = 0; s < n; ++s) {
+        src[s].img = imgs[s];
+        src[s].h = img_hw[2 * s];
+        src[s].w = img_hw[2 * s + 1];
+        memcpy(src[s].geo, geo + 18 * s, sizeof(double) * 18);
+    }
+    om_sweep_desc d;
+    memset(&d, 0, sizeof d);
+    d.ref = ref;
+    d.mu = mu;
+    d.sd = sd;
+    d.ray = ray;
+    d.H = H;
+    d.W = W;
+    d.n_src = n;
+    d.src = src;
+    d.r = r;
+    d.top_k = top_k;
+    d.hyp_mode = OM_HYP_MAPS;
+    d.hyp_a = inv;
+    d.i0 = i;
+    d.nh = 1;
+    d.best = best;
+    d.prev = prev;
+    d.s_prev_best = s_prev_best;
+    d.s_next_best = s_next_best;
+    d.idx = idx;
+    d.chunks = 1;
+    return om_sweep(&d, stream);
+}
+
+extern "C" int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *stream)
+{
+    SweepK P;
+    int rc = build_params(d, P);
+    if (rc)
+        return rc;
+    if (d->i0 != 0)
+        return fail(OM_EINVAL, "om_sweep_finalize expects the full hypothesis range (i0 = 0)");
+    if (d->hyp_mode == OM_HYP_MAPS && d->nh > 1 && !d->hyp_a)
+        return fail(OM_EINVAL, "null hypothesis maps");
+    const int64_t HW = (int64_t)P.H * P.W;
+    if (HW == 0)
+        return OM_OK;
+    hipLaunchKernelGGL(sweep_finalize_kernel, dim3((unsigned)cdiv(HW, 256)), dim3(256), 0, as_stream(stream), P,
+                       depth);
+    return launch_check("sweep_finalize_kernel");
+}
