@@ -1,0 +1,277 @@
+#!/usr/bin/env python
+"""Throughput of the per-view depth-map path on one MI355X.
+
+Workload (default, the headline configuration): single-view PatchMatch at drone scale -- a
+2000x1500 reference view with 8 source views on a 2x4 grid (20 m baseline, 100 m altitude, f=1800)
+of the seeded synthetic terrain, seeded random slanted-plane start (depth U(80,105) m, normals
+within 30 deg of up), 11x11 window sampled every 2 px (36 samples), 8 red-black iterations, geometric
+term against the sources' rendered depth maps.  One step = one full PatchMatch refinement of the
+view (_dense.pm_refine semantics) from the seeded start, inputs resident in HBM.
+
+Metric: Mpixel-hypotheses/s = H * W * 11 * iterations / step time (11 cost evaluations per pixel
+and iteration: current + 8 propagated + 2 random).  Also printed: p50/p99 step latency, the
+dominant kernel's roofline (HIP events around its launches), and the oracle (C restatement of the
+reference) timed on this host's cores on a bounded centre crop of the same workload.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c2|c1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpixel-hypotheses/s on 1 GPU; p50/p99 depth-map latency at fixed accuracy"
+FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
+HBM_PEAK = 8.0e12
+
+
+def pm_flops_per_eval(S: int, nsamp: int) -> int:
+    """Algorithmic FLOPs of one PatchMatch cost evaluation (FMA = 2): per sample per source
+    homography apply 15 + bilinear 11 + 3 accumulators 5, a shared ref term 3 per sample, and per
+    source plane->homography 60 + NCC 15 + geometric term 19."""
+    return nsamp * (31 * S + 3) + 94 * S + 5
+
+
+def sweep_flops_per_hyp(S: int) -> int:
+    """Per pixel-hypothesis: 55 per source (homography 15, bilinear 11, J^2 / J*I 2, separable box
+    sums 16, NCC 11) + 6 (hypothesis 2, top-k mean 3, winner 1)."""
+    return 55 * S + 6
+
+
+def _cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def _percentile(v, q):
+    return float(np.percentile(np.asarray(v), q)) if v else None
+
+
+def prof_read(lib, name):
+    tot, n = ctypes.c_double(0), ctypes.c_int64(0)
+    rc = lib.om_profile_read(name.encode(), ctypes.byref(tot), ctypes.byref(n))
+    if rc:
+        raise RuntimeError(f"om_profile_read rc={rc}")
+    return tot.value, n.value
+
+
+# ------------------------------------------------------------------------------- C2 PatchMatch
+def setup_c2(torch, scale=1.0):
+    from py_ortho_amd import scene
+    sc = scene.config_c2(device="cuda", scale=scale)
+    H, W = sc.ref.img.shape
+    ray = scene.rays_world(sc.ref.cam, "cuda")
+    depth0, normal0 = scene.pm_init(3, H, W, ray, 80.0, 105.0)
+    return dict(scene=sc, H=H, W=W, ref=sc.ref.img.contiguous(), ray=ray,
+                srcs=[v.img.contiguous() for v in sc.srcs], deps=[v.depth.contiguous() for v in sc.srcs],
+                par=sc.par(), Cr=sc.ref.cam.C.copy(), depth0=depth0.cuda(), normal0=normal0.cuda(),
+                active=torch.ones((H, W), dtype=torch.uint8, device="cuda"),
+                r=5, step=2, top_k=2, geo_w=0.5, geo_tol=0.01, iters=8, dmin=80.0, dmax=105.0,
+                seed=2654435761)
+
+
+def run_c2(args, torch):
+    from py_ortho_amd import _lib, dense
+    lib = _lib.lib()
+    c = setup_c2(torch)
+    H, W, S = c["H"], c["W"], len(c["srcs"])
+    depth = torch.empty_like(c["depth0"])
+    normal = torch.empty_like(c["normal0"])
+    score = torch.empty((H, W), dtype=torch.float32, device="cuda")
+
+    def step():
+        depth.copy_(c["depth0"])
+        normal.copy_(c["normal0"])
+        score.fill_(-1.0)
+        return dense.pm_refine(c["ref"], c["ray"], c["Cr"], c["srcs"], c["deps"], c["par"], depth, normal, score,
+                               c["active"], c["r"], c["step"], c["top_k"], c["geo_w"], c["geo_tol"], c["iters"],
+                               c["dmin"], c["dmax"], c["seed"])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    prof_read(lib, "pm_propagate")  # drop warm-up records
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        changed = step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lib.om_profile_enable(0)
+    prop_ms, prop_n = prof_read(lib, "pm_propagate")
+    rand_ms, rand_n = prof_read(lib, "pm_random")
+    lib.om_profile_read(None, None, None)
+
+    evals = H * W * 11 * c["iters"]
+    ms = wall / args.steps * 1e3
+    nsamp = ((2 * c["r"]) // c["step"] + 1) ** 2
+    fpe = pm_flops_per_eval(S, nsamp)
+    # dominant kernel: propagation, 9 nominal evaluations per pixel of one colour
+    prop_avg = prop_ms / max(prop_n, 1) / 1e3
+    prop_flops = (H * W / 2) * 9 * fpe
+    achieved = prop_flops / prop_avg if prop_avg > 0 else 0.0
+    # accuracy of the refined map against the rendered truth
+    truth = c["scene"].ref.depth
+    rel = ((depth - truth).abs() / truth).float()
+    fin = torch.isfinite(rel)
+    acc = {"frac_within_1pct": float((rel[fin] < 0.01).float().mean()),
+           "median_rel_err": float(rel[fin].median()), "changed_last_step": int(changed)}
+    res = {
+        "metric": METRIC, "value": evals / (ms / 1e3) / 1e6, "unit": "Mpixel-hyp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C2 single-view PatchMatch, 2000x1500 ref, 8 sources (2x4 grid, 20 m, 100 m alt, "
+                               "f=1800), 11x11 window step 2 (36 samples), 8 red-black iterations, geo term",
+                   "H": H, "W": W, "sources": S, "iterations": c["iters"], "pixel_hypotheses_per_step": evals},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)", "achieved": achieved / 1e12,
+                     "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None,
+                     "kernel": "pm_propagate_kernel", "kernel_avg_ms": prop_avg * 1e3, "launches": prop_n,
+                     "flops_per_launch": prop_flops, "flops_per_eval": fpe,
+                     "pm_random_avg_ms": rand_ms / max(rand_n, 1)},
+        "accuracy": acc,
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_pm(c, torch, args)
+    return res
+
+
+def cpu_baseline_pm(c, torch, args):
+    """The oracle's pm_refine (C restatement of _dense.pm_refine, one view per thread like
+    densify.py:596) on a centre crop of the reference with the full sources, `cores` concurrent
+    copies; Mpixel-hyp/s = copies * crop pixels * 11 * iterations / wall."""
+    import oracle
+    H, W = c["H"], c["W"]
+    ref = c["ref"].cpu().numpy()
+    ray = c["ray"].cpu().numpy()
+    d0 = c["depth0"].cpu().numpy()
+    n0 = c["normal0"].cpu().numpy()
+    srcs = [s.cpu().numpy() for s in c["srcs"]]
+    deps = [d.cpu().numpy() for d in c["deps"]]
+    cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
+
+    def crop(side, iters, threads):
+        y0, x0 = (H - side) // 2, (W - side) // 2
+        sl = np.s_[y0:y0 + side, x0:x0 + side]
+        a = np.ones((side, side), np.uint8)
+        t = time.perf_counter()
+        oracle.bench_pm(threads, ref[sl], np.ascontiguousarray(ray[:, y0:y0 + side, x0:x0 + side]), c["Cr"], srcs,
+                        deps, c["par"], np.ascontiguousarray(d0[sl]), np.ascontiguousarray(n0[:, y0:y0 + side,
+                                                                                              x0:x0 + side]),
+                        a, c["r"], c["step"], c["top_k"], c["geo_w"], c["geo_tol"], iters, c["dmin"], c["dmax"],
+                        c["seed"])
+        return time.perf_counter() - t
+
+    probe = crop(16, 1, 1)                         # seconds per 256 pixel-iterations, one core
+    per_px_it = probe / 256.0
+    target = 1.5                                   # seconds of work per core (bounded sample)
+    side = int(max(16, min(256, math.sqrt(target / (per_px_it * c["iters"])))))
+    wall = crop(side, c["iters"], cores)
+    evals = cores * side * side * 11 * c["iters"]
+    return {"value": evals / wall / 1e6, "unit": "Mpixel-hyp/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent copies of the oracle pm_refine on the {side}x{side} centre crop, "
+                      f"S={len(srcs)}, {c['iters']} iterations ({wall:.1f} s wall)",
+            "single_core_latency_s_per_mpix": wall / (side * side) * 1e6}
+
+
+# ------------------------------------------------------------------------------- C1 plane sweep
+def run_c1(args, torch):
+    from py_ortho_amd import _lib, dense, scene
+    lib = _lib.lib()
+    sc = scene.config_c1(device="cuda")
+    H, W = sc.ref.img.shape
+    r, k, D = 2, 2, 128
+    ref = sc.ref.img.contiguous()
+    ray = scene.rays_world(sc.ref.cam, "cuda")
+    mu, sd = dense.ref_stats(ref, r)
+    planes = torch.linspace(60.0, 85.0, D, dtype=torch.float64)
+    inv = (1.0 / planes).to(torch.float32).cuda()
+    srcs = [v.img.contiguous() for v in sc.srcs]
+    geo = sc.geo()
+    hyp = {"mode": "const", "inv": inv}
+
+    def step():
+        st = dense.sweep(ref, mu, sd, srcs, ray, geo, hyp, r, k)
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    prof_read(lib, "sweep")
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        st = step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    lib.om_profile_enable(0)
+    sw_ms, sw_n = prof_read(lib, "sweep")
+    lib.om_profile_read(None, None, None)
+    ms = wall / args.steps * 1e3
+    S = len(srcs)
+    avg = sw_ms / max(sw_n, 1) / 1e3
+    flops = H * W * D * sweep_flops_per_hyp(S)
+    truth = sc.ref.depth
+    dw = planes.cuda()[st.idx.long()].float()
+    rel = (dw - truth).abs() / truth
+    valid = (st.best >= 0.3) & (255.0 * sd >= 0.5)
+    res = {
+        "metric": METRIC, "value": H * W * D / (ms / 1e3) / 1e6, "unit": "Mpixel-hyp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C1 plane sweep 640x480, 4 sources, 128 fronto-parallel planes 60-85 m, 5x5 NCC, "
+                               "top-2, WTA", "H": H, "W": W, "sources": S, "planes": D},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "roofline": {"bound": "mfma", "engine": "VALU FP64", "achieved": flops / avg / 1e12 if avg else 0,
+                     "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": (flops / avg / FP64_PEAK) if avg else 0,
+                     "traffic": None, "kernel": "sweep_kernel", "kernel_avg_ms": avg * 1e3},
+        "accuracy": {"valid_frac": float(valid.float().mean()),
+                     "valid_within_1pct": float((rel[valid] < 0.01).float().mean())},
+    }
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = all cores)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (views shard across "
+              "GPUs without communication; see the sharding notes)", file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    res = run_c2(args, torch) if args.workload == "c2" else run_c1(args, torch)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -79,6 +79,10 @@ typedef struct om_sweep_desc {
 
 /* library */
 int om_version(void);
+/* measurement: when on, every kernel launch is bracketed by HIP events on its stream;
+ * om_profile_read sums (and forgets) the recorded launches of `kernel` (NULL = all kernels). */
+int om_profile_enable(int32_t on);
+int om_profile_read(const char *kernel, double *total_ms, int64_t *launches);
 const char *om_last_error(void);
 /* name (>= 64 bytes), compute units, clock kHz, global memory bytes of the current device */
 int om_device_info(char *name, int32_t name_len, int32_t *cus, int32_t *clock_khz, int64_t *mem_bytes);
@@ -121,6 +125,25 @@ size_t om_box_workspace_bytes(int32_t H, int32_t W, int32_t r);
 int om_ref_stats(const float *ref, int32_t H, int32_t W, int32_t r, float *mu, float *sd, void *workspace,
                  size_t workspace_bytes, void *stream);
 int om_half(const float *g, int32_t H, int32_t W, float *out, void *stream);
+/* densify._rays_world (densify.py:288-297): (3, H, W) float32 world rays, camera-axis depth 1;
+ * R: host row-major world->camera rotation. */
+int om_rays_world(int32_t H, int32_t W, const double *R, double cx, double cy, double f, double k1, double k2,
+                  double k3, float *out, void *stream);
+
+/* height-sweep backend operations (orthomosaic/backend.py:67-99) */
+/* _mvs.sample_view (_mvs.pyx:13-54) over `planes` height maps Z (planes, H, W): img (h, w, nch)
+ * float32; out (planes, H, W, nch) must be zeroed by the caller (untouched where invalid). */
+int om_sample_view(const float *img, int32_t h, int32_t w, int32_t nch, const double *R, const double *C, double f,
+                   double k1, double k2, double k3, double cx, double cy, double X0, double Y0, double gsd,
+                   const float *Z, int64_t planes, int32_t H, int32_t W, float *out, uint8_t *valid, void *stream);
+/* _fast.box_sat_mean (_fast.pyx:897-943) of B float32 planes (B, H, W) */
+size_t om_box_sat_workspace_bytes(int64_t B, int32_t H, int32_t W, int32_t r);
+int om_box_sat_mean(const float *a, int64_t B, int32_t H, int32_t W, int32_t r, float *out, void *workspace,
+                    size_t workspace_bytes, void *stream);
+/* CPUBackend.topk_mean: mean of the k largest of st (n, M) along axis 0 (k <= 16) */
+int om_topk_mean(const float *st, int32_t n, int64_t M, int32_t k, float *out, void *stream);
+/* CPUBackend.argmin0 / argmax0: first index (int64) and value along axis 0 of a (n, M) */
+int om_argminmax0(const float *a, int32_t n, int64_t M, int32_t want_max, int64_t *idx, float *val, void *stream);
 
 #ifdef __cplusplus
 }

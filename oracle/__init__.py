@@ -227,6 +227,12 @@ def rays_world(H, W, R, cx, cy, f, k1=0.0, k2=0.0, k3=0.0):
     return np.ascontiguousarray(np.einsum("ji,jhw->ihw", np.asarray(R, np.float64), ray_c), np.float32)
 
 
+def xorshift32_step(y):
+    L = orc()
+    L.orc_xorshift32_step.restype = ctypes.c_uint32
+    return int(L.orc_xorshift32_step(ctypes.c_uint32(y & 0xFFFFFFFF)))
+
+
 def xorshift_draws(seed, n):
     out = np.empty(n, np.float64)
     orc().orc_xorshift_draws(ctypes.c_uint32(seed & 0xFFFFFFFF), ctypes.c_int64(n), _ptr(out, _f64p))
@@ -294,3 +300,32 @@ def bench_pm(nthreads, ref, ray, Cr, srcs, src_depths, par, depth0, normal0, act
                             ctypes.c_double(dmin), ctypes.c_double(dmax), ctypes.c_uint32(int(seed) & 0xFFFFFFFF))
     if rc:
         raise RuntimeError(f"bench_pm rc={rc}")
+
+
+def sample_view(img, cam, X0, Y0, gsd, Z):
+    """Restatement of _mvs.sample_view (_mvs.pyx:13-54) over a (..., H, W) stack of heights, as
+    CPUBackend.sample_view calls it (backend.py:67-77) -> (out (..., H, W, ch), valid uint8)."""
+    R, C, f, k1, k2, k3, cx, cy = cam
+    img = _c32(img)
+    if img.ndim == 2:
+        img = img[:, :, None]
+    Z = _c32(Z)
+    R = np.ascontiguousarray(R, np.float64)
+    C = np.ascontiguousarray(C, np.float64)
+    h, w, nch = img.shape
+    out = np.zeros(Z.shape + (nch,), np.float32)
+    valid = np.zeros(Z.shape, np.uint8)
+    Zs = Z.reshape((-1,) + Z.shape[-2:])
+    os_ = out.reshape((-1,) + out.shape[-3:])
+    vs = valid.reshape((-1,) + Z.shape[-2:])
+    H, W = Z.shape[-2:]
+    for d in range(Zs.shape[0]):
+        zd = np.ascontiguousarray(Zs[d])
+        od = np.zeros((H, W, nch), np.float32)
+        vd = np.zeros((H, W), np.uint8)
+        orc().orc_sample_view(_ptr(img, _f32p), ctypes.c_int64(h), ctypes.c_int64(w), ctypes.c_int64(nch),
+                              _ptr(R, _f64p), _ptr(C, _f64p), *(ctypes.c_double(float(v)) for v in
+                                                                 (f, k1, k2, k3, cx, cy, X0, Y0, gsd)),
+                              _ptr(zd, _f32p), ctypes.c_int64(H), ctypes.c_int64(W), _ptr(od, _f32p), _ptr(vd, _u8p))
+        os_[d], vs[d] = od, vd
+    return out, valid

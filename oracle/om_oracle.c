@@ -435,6 +435,44 @@ ORC_EXPORT void orc_half(const float *g, int64_t H, int64_t W, float *out)
         }
 }
 
+/* Ground cell (X0 + (c+.5) gsd, Y0 - (r+.5) gsd, Z[r,c]) -> camera -> bilinear sample of an
+ * (h, w, nch) image with edge clamp: _mvs.sample_view, _mvs.pyx:13-54.  out must be zeroed by the
+ * caller (the reference's CPU backend allocates it with np.zeros, backend.py:71). */
+ORC_EXPORT void orc_sample_view(const float *img, int64_t h, int64_t w, int64_t nch, const double *R, const double *C,
+                                double f, double k1, double k2, double k3, double cx, double cy, double X0, double Y0,
+                                double gsd, const float *Z, int64_t H, int64_t W, float *out, uint8_t *valid)
+{
+    for (int64_t r = 0; r < H; r++) {
+        const double Y = Y0 - ((double)r + 0.5) * gsd;
+        for (int64_t c = 0; c < W; c++) {
+            const double X = X0 + ((double)c + 0.5) * gsd;
+            const double dx = X - C[0], dy = Y - C[1], dz = (double)Z[r * W + c] - C[2];
+            const double zc = R[6] * dx + R[7] * dy + R[8] * dz;
+            valid[r * W + c] = 0;
+            if (zc <= 1e-6)
+                continue;
+            const double xc = R[0] * dx + R[1] * dy + R[2] * dz;
+            const double yc = R[3] * dx + R[4] * dy + R[5] * dz;
+            const double nx = xc / zc, ny = yc / zc;
+            const double q = nx * nx + ny * ny;
+            const double d = 1.0 + q * (k1 + q * (k2 + q * k3));
+            const double u = f * d * nx + cx, v = f * d * ny + cy;
+            if (!(u >= 0 && v >= 0 && u <= (double)(w - 1) && v <= (double)(h - 1)))
+                continue;
+            const int64_t x0 = (int64_t)u, y0 = (int64_t)v;
+            const int64_t x1 = x0 + 1 < w ? x0 + 1 : x0, y1 = y0 + 1 < h ? y0 + 1 : y0;
+            const double fx = u - (double)x0, fy = v - (double)y0;
+            for (int64_t ch = 0; ch < nch; ch++)
+                out[(r * W + c) * nch + ch] =
+                    (float)((1 - fy) * ((1 - fx) * (double)img[(y0 * w + x0) * nch + ch] +
+                                        fx * (double)img[(y0 * w + x1) * nch + ch]) +
+                            fy * ((1 - fx) * (double)img[(y1 * w + x0) * nch + ch] +
+                                  fx * (double)img[(y1 * w + x1) * nch + ch]));
+            valid[r * W + c] = 1;
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* PatchMatch repair: _dense.pyx:405-666                                                        */
 /* ------------------------------------------------------------------------------------------ */
@@ -711,6 +749,13 @@ ORC_EXPORT int64_t orc_pm_refine(const float *ref, int H, int W, const float *ra
         }
     }
     return changed;
+}
+
+/* one raw step of the stream behind pm_rnd (tests pin it against Marsaglia's published value) */
+ORC_EXPORT uint32_t orc_xorshift32_step(uint32_t y)
+{
+    (void)pm_rnd(&y);
+    return y;
 }
 
 /* xorshift32 stream helper for tests: n successive draws of pm_rnd from `seed | 1`. */

@@ -46,6 +46,8 @@ class OmSweepDesc(ctypes.Structure):
 SIGNATURES = {
     "om_version": (c_int32, []),
     "om_last_error": (ctypes.c_char_p, []),
+    "om_profile_enable": (c_int32, [c_int32]),
+    "om_profile_read": (c_int32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int64)]),
     "om_device_info": (c_int32, [ctypes.c_char_p, c_int32, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
                                  ctypes.POINTER(c_int64)]),
     "om_sweep_workspace_bytes": (c_size_t, [ctypes.POINTER(OmSweepDesc)]),
@@ -68,6 +70,17 @@ SIGNATURES = {
     "om_ref_stats": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_size_t,
                                c_void_p]),
     "om_half": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
+    "om_rays_world": (c_int32, [c_int32, c_int32, ctypes.POINTER(c_double), c_double, c_double, c_double, c_double,
+                                c_double, c_double, c_void_p, c_void_p]),
+    "om_sample_view": (c_int32, [c_void_p, c_int32, c_int32, c_int32, ctypes.POINTER(c_double),
+                                 ctypes.POINTER(c_double), c_double, c_double, c_double, c_double, c_double, c_double,
+                                 c_double, c_double, c_double, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p,
+                                 c_void_p]),
+    "om_box_sat_workspace_bytes": (c_size_t, [c_int64, c_int32, c_int32, c_int32]),
+    "om_box_sat_mean": (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_size_t,
+                                  c_void_p]),
+    "om_topk_mean": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p, c_void_p]),
+    "om_argminmax0": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -3,15 +3,33 @@
 // Reference semantics: orthomosaic/_fast.pyx:628-713 (hyp_linear, hyp_refine, box_zero_mean) and
 // orthomosaic/densify.py:127-130 (_half), :327-328 (mu / sd of _sweep).  These helpers reproduce
 // the reference's float32 rounding sequence exactly (the library is built with -ffp-contract=off).
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "om_common.h"
 
 namespace om {
 
 static thread_local char g_err[512] = "";
+
+static std::atomic<int> g_prof{0};
+static std::mutex g_prof_mu;
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+static std::vector<ProfRec> g_prof_recs;
+
+bool prof_on() { return g_prof.load(std::memory_order_relaxed) != 0; }
+
+void prof_push(const char *name, hipEvent_t a, hipEvent_t b)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_recs.push_back({name, a, b});
+}
 
 void set_error(const char *fmt, ...)
 {
@@ -120,6 +138,44 @@ __global__ void box_out_kernel(const float *rows, const float *rows2, int H, int
     }
 }
 
+// World ray of every pixel with camera-axis depth 1 (densify._rays_world, densify.py:288-297):
+// 8 fixed-point undistortion steps in FP64 (_fast.pyx:145-164, one rounding per operation), then
+// R^T (x, y, 1) summed over j = 0, 1, 2 in order, rounded to float32.
+struct RaysK {
+    double R[9];
+    double cx, cy, f, k1, k2, k3;
+    int H, W;
+};
+
+__global__ void rays_world_kernel(const RaysK P, float *out)
+{
+    const int64_t HW = (int64_t)P.H * P.W;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= HW)
+        return;
+    const int v = (int)(p / P.W), u = (int)(p - (int64_t)v * P.W);
+    const double nx = ((double)u - P.cx) / P.f, ny = ((double)v - P.cy) / P.f;
+    double x = nx, y = ny;
+    for (int it = 0; it < 8; ++it) {
+        double q = x * x;
+        const double t = y * y;
+        q = q + t;
+        double d = q * P.k3;
+        d = P.k2 + d;
+        d = q * d;
+        d = P.k1 + d;
+        d = q * d;
+        d = 1.0 + d;
+        x = nx / d;
+        y = ny / d;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double a = P.R[i] * x, b = P.R[3 + i] * y, c = P.R[6 + i] * 1.0;
+        out[i * HW + p] = (float)((a + b) + c);
+    }
+}
+
 static size_t box_tables_bytes(int H, int W, int r)
 {
     const int64_t k = 2 * (int64_t)r + 1;
@@ -131,6 +187,42 @@ static size_t box_tables_bytes(int H, int W, int r)
 using namespace om;
 
 extern "C" int om_version(void) { return 10000; }
+
+extern "C" int om_profile_enable(int32_t on)
+{
+    g_prof.store(on ? 1 : 0);
+    return OM_OK;
+}
+
+extern "C" int om_profile_read(const char *kernel, double *total_ms, int64_t *launches)
+{
+    double tot = 0.0;
+    int64_t n = 0;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    std::vector<ProfRec> keep;
+    for (const ProfRec &r : g_prof_recs) {
+        if (kernel && strcmp(kernel, r.name) != 0) {
+            keep.push_back(r);
+            continue;
+        }
+        float ms = 0.f;
+        int rc = hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+        if (!rc)
+            rc = hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+        if (rc)
+            return rc;
+        tot += ms;
+        n += 1;
+    }
+    g_prof_recs.swap(keep);
+    if (total_ms)
+        *total_ms = tot;
+    if (launches)
+        *launches = n;
+    return OM_OK;
+}
 
 extern "C" const char *om_last_error(void) { return g_err; }
 
@@ -186,6 +278,31 @@ extern "C" int om_half(const float *g, int32_t H, int32_t W, float *out, void *s
         return OM_OK;
     hipLaunchKernelGGL(half_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, as_stream(stream), g, H, W, out);
     return launch_check("half_kernel");
+}
+
+extern "C" int om_rays_world(int32_t H, int32_t W, const double *R, double cx, double cy, double f, double k1,
+                             double k2, double k3, float *out, void *stream)
+{
+    if (H < 0 || W < 0)
+        return fail(OM_ESHAPE, "negative size");
+    if (!R)
+        return fail(OM_EINVAL, "null rotation");
+    const int64_t n = (int64_t)H * W;
+    if (n == 0)
+        return OM_OK;
+    RaysK P;
+    for (int i = 0; i < 9; ++i)
+        P.R[i] = R[i];
+    P.cx = cx;
+    P.cy = cy;
+    P.f = f;
+    P.k1 = k1;
+    P.k2 = k2;
+    P.k3 = k3;
+    P.H = H;
+    P.W = W;
+    hipLaunchKernelGGL(rays_world_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, as_stream(stream), P, out);
+    return launch_check("rays_world_kernel");
 }
 
 extern "C" size_t om_box_workspace_bytes(int32_t H, int32_t W, int32_t r)

@@ -38,4 +38,38 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Optional per-launch timing (om_profile_enable): HIP events recorded on the launch stream
+// around each instrumented kernel, read back by om_profile_read.  Off by default; when off a
+// scope costs one relaxed atomic load.
+bool prof_on();
+void prof_push(const char *name, hipEvent_t a, hipEvent_t b);
+
+struct ProfScope {
+    const char *name = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(const char *n, hipStream_t s)
+    {
+        if (!prof_on())
+            return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            return;
+        name = n;
+        st = s;
+        if (hipEventRecord(a, st) != hipSuccess)
+            name = nullptr;
+    }
+    ~ProfScope()
+    {
+        if (name && hipEventRecord(b, st) == hipSuccess)
+            prof_push(name, a, b);
+        else {
+            if (a)
+                (void)hipEventDestroy(a);
+            if (b)
+                (void)hipEventDestroy(b);
+        }
+    }
+};
+
 }  // namespace om

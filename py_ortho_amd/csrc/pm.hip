@@ -1,14 +1,628 @@
-// PatchMatch repair (placeholder until the kernel lands).
+// Red-black slanted-plane PatchMatch repair for gfx950.
+//
+// Reference semantics: _dense.pm_refine and its helpers (orthomosaic/_dense.pyx:405-666):
+// PMCtx :408, _proj :429-445, _bil :448-458, _pm_cost :461-556, _rnd :559-563, the pass loop
+// :606-664.
+//
+// Parallel form of a sequential algorithm.  Within one colour pass every pixel reads only pixels
+// of the other colour (all propagation offsets are odd, _dense.pyx:583-586) and its own state, so
+// the pixels of a pass are independent except for the single xorshift32 stream, which the
+// reference consumes in row-major order: 8 draws per active pixel whose depth is finite after
+// propagation, nothing for the others (:647-655).  A pass therefore runs as
+//   A  pm_propagate  one thread per pixel of the colour: current cost + 8 propagated planes,
+//   B  pm_rank_rows / pm_scan_rows  each drawing pixel's rank in row-major order, the pass total,
+//      and the stream state after the pass,
+//   C  pm_random     the stream jumped ahead to the pixel's offset (8 * rank) with precomputed
+//                    GF(2) powers of the xorshift32 step matrix, then the 2 random tries.
+// The draws each pixel sees are exactly the reference's, so the result equals the sequential one.
+//
+// Arithmetic.  Costs are evaluated in FP64 with the reference's operation order (the library is
+// compiled with -ffp-contract=off), accumulating the window sums per source in sample order, so
+// every accept/reject decision is the reference's.  One documented difference: when a cost
+// evaluation returns early (plane facing away from the ray, or d <= 0) the reference leaves its
+// photometric score variable holding the previous evaluation's value, possibly another pixel's;
+// here it holds -1 at the start of each pixel.
+#include <cmath>
+#include <cstring>
+
 #include "om_common.h"
+
+namespace om {
+
+// ---------------------------------------------------------------- xorshift32 jump-ahead
+struct XsJump {
+    uint32_t m[64][32];  // m[j][c] = M^(2^j) e_c, M = the xorshift32 step (13, 17, 5) over GF(2)
+};
+
+constexpr uint32_t xs_step(uint32_t s)
+{
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
+}
+
+constexpr uint32_t xs_apply_cols(const uint32_t (&cols)[32], uint32_t s)
+{
+    uint32_t v = 0;
+    for (int c = 0; c < 32; ++c)
+        if ((s >> c) & 1u)
+            v ^= cols[c];
+    return v;
+}
+
+constexpr XsJump make_xs_jump()
+{
+    XsJump t{};
+    for (int c = 0; c < 32; ++c)
+        t.m[0][c] = xs_step(1u << c);
+    for (int j = 1; j < 64; ++j)
+        for (int c = 0; c < 32; ++c)
+            t.m[j][c] = xs_apply_cols(t.m[j - 1], t.m[j - 1][c]);
+    return t;
+}
+
+__constant__ XsJump kXsJump = make_xs_jump();
+
+__device__ inline uint32_t xs_jump(uint32_t s, uint64_t n)
+{
+    for (int j = 0; j < 64 && n; ++j, n >>= 1) {
+        if (n & 1u) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                v ^= kXsJump.m[j][c] & (0u - ((s >> c) & 1u));
+            s = v;
+        }
+    }
+    return s;
+}
+
+__device__ inline double xs_rnd(uint32_t &st)
+{
+    st ^= st << 13;
+    st ^= st >> 17;
+    st ^= st << 5;
+    return (double)(st & 0xFFFFFFu) / 16777216.0;
+}
+
+// ---------------------------------------------------------------- kernel arguments
+struct PMK {
+    const float *ref, *ray;
+    const uint8_t *active;
+    float *depth, *normal, *score;
+    int H, W, S, r, step, kk, pinhole, colour;
+    double cr0, cr1, cr2, geo_w, geo_tol, dmin, dmax, dsc, nsc;
+    double *best, *sc;         // (H, W) per-pixel cost and photometric score of the pass
+    int *rank;                 // (H, W) draw rank of the pixel within its row, -1 = no draws
+    int *row_cnt, *row_off;    // (H,)
+    uint32_t *st_in, *st_out;  // stream state before / after the pass
+    unsigned long long *changed;
+    const float *img[16];
+    const float *dep[16];
+    int ih[16], iw[16];
+    double par[16][18];
+};
+
+// world point -> source pixel (_dense.pyx:429-445)
+__device__ __forceinline__ bool pm_proj(const PMK &P, int s, double X, double Y, double Z, double &u, double &v,
+                                        double &z)
+{
+    const double *p = P.par[s];
+    const double dx = X - p[9], dy = Y - p[10], dz = Z - p[11];
+    const double xc = p[0] * dx + p[1] * dy + p[2] * dz;
+    const double yc = p[3] * dx + p[4] * dy + p[5] * dz;
+    const double zc = p[6] * dx + p[7] * dy + p[8] * dz;
+    if (zc <= 1e-6)
+        return false;
+    const double nx = xc / zc, ny = yc / zc;
+    double dd;
+    if (P.pinhole) {
+        dd = p[12];  // f * (1 + r2 * 0) == f for finite r2
+    } else {
+        const double q = nx * nx + ny * ny;
+        dd = p[12] * (1.0 + q * (p[13] + q * (p[14] + q * p[15])));
+    }
+    u = dd * nx + p[16];
+    v = dd * ny + p[17];
+    z = zc;
+    return true;
+}
+
+// bilinear sample, NaN outside [0, w-1.001] x [0, h-1.001] (_dense.pyx:448-458)
+__device__ __forceinline__ double pm_bil(const float *im, int w, int h, double u, double v)
+{
+    if (!(u >= 0.0 && v >= 0.0 && u <= (double)w - 1.001 && v <= (double)h - 1.001))
+        return NAN;
+    const int x0 = (int)u, y0 = (int)v;
+    const double fx = u - (double)x0, fy = v - (double)y0;
+    const float *p = im + (size_t)y0 * w + x0;
+    return ((double)p[0] * (1.0 - fx) + (double)p[1] * fx) * (1.0 - fy) +
+           ((double)p[w] * (1.0 - fx) + (double)p[w + 1] * fx) * fy;
+}
+
+// slanted-window cost (_dense.pyx:461-556); photo is written only on the full path
+template <int SM>
+__device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, double n1, double n2, double &photo)
+{
+    const int H = P.H, W = P.W;
+    const size_t HW = (size_t)H * W, p = (size_t)y * W + x;
+    const double rpx = P.ray[p], rpy = P.ray[HW + p], rpz = P.ray[2 * HW + p];
+    const double ndr = n0 * rpx + n1 * rpy + n2 * rpz;
+    if (ndr >= -1e-6 || d <= 0)
+        return 10.0;
+    const double num = d * ndr;
+    double sI[SM], sJ[SM], sII[SM], sJJ[SM], sIJ[SM];
+    int cnt[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+        sI[s] = sJ[s] = sII[s] = sJJ[s] = sIJ[s] = 0.0;
+        cnt[s] = 0;
+    }
+    int nsamp = 0;
+    const int r = P.r, st = P.step, S = P.S;
+    for (int a = -r; a <= r; a += st) {
+        const int yy = y + a;
+        if (yy < 0 || yy >= H)
+            continue;
+        for (int b = -r; b <= r; b += st) {
+            const int xx = x + b;
+            if (xx < 0 || xx >= W)
+                continue;
+            const size_t q = (size_t)yy * W + xx;
+            const double I = P.ref[q];
+            if (!isfinite(I))
+                continue;
+            const double qx = P.ray[q], qy = P.ray[HW + q], qz = P.ray[2 * HW + q];
+            const double den = n0 * qx + n1 * qy + n2 * qz;
+            if (den >= -1e-9)
+                continue;
+            const double t = num / den;
+            const double X = P.cr0 + t * qx, Y = P.cr1 + t * qy, Z = P.cr2 + t * qz;
+            nsamp += 1;
+#pragma unroll
+            for (int s = 0; s < SM; ++s) {
+                if (s < S) {
+                    double u, v, z;
+                    if (pm_proj(P, s, X, Y, Z, u, v, z)) {
+                        const double J = pm_bil(P.img[s], P.iw[s], P.ih[s], u, v);
+                        if (isfinite(J)) {
+                            sI[s] += I;
+                            sJ[s] += J;
+                            sII[s] += I * I;
+                            sJJ[s] += J * J;
+                            sIJ[s] += I * J;
+                            cnt[s] += 1;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    double ncc[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+        if (s < S) {
+            if (nsamp < 4 || cnt[s] < 0.7 * nsamp) {
+                ncc[s] = -1.0;
+            } else {
+                const double m = (double)cnt[s];
+                const double mi = sI[s] / m, mj = sJ[s] / m;
+                const double vi = sII[s] / m - mi * mi;
+                const double vj = sJJ[s] / m - mj * mj;
+                const double cv = sIJ[s] / m - mi * mj;
+                ncc[s] = (vi < 1e-8 || vj < 1e-8) ? 0.0 : cv / sqrt(vi * vj);
+            }
+        } else {
+            ncc[s] = 0.0;
+        }
+    }
+    // descending insertion sort of the S values, the reference's exact procedure (:520-526)
+#pragma unroll
+    for (int a = 1; a < SM; ++a) {
+        if (a < S) {
+            const double t = ncc[a];
+            bool moving = true;
+#pragma unroll
+            for (int c = a - 1; c >= 0; --c) {
+                if (moving) {
+                    if (ncc[c] < t) {
+                        ncc[c + 1] = ncc[c];
+                    } else {
+                        ncc[c + 1] = t;
+                        moving = false;
+                    }
+                }
+            }
+            if (moving)
+                ncc[0] = t;
+        }
+    }
+    double ph = 0.0;
+#pragma unroll
+    for (int a = 0; a < SM; ++a)
+        if (a < P.kk)
+            ph += ncc[a];
+    ph /= P.kk;
+    photo = ph;
+    double geo = 0.0;
+    if (P.geo_w > 0) {
+        const double X = P.cr0 + d * rpx, Y = P.cr1 + d * rpy, Z = P.cr2 + d * rpz;
+        int ng = 0;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+            if (s < S && P.dep[s]) {
+                ng += 1;
+                double e = 1.0, u, v, z;
+                if (pm_proj(P, s, X, Y, Z, u, v, z)) {
+                    const int ui = (int)(u + 0.5), vi = (int)(v + 0.5);
+                    if (ui >= 0 && vi >= 0 && ui < P.iw[s] && vi < P.ih[s]) {
+                        const double dj = P.dep[s][(size_t)vi * P.iw[s] + ui];
+                        if (isfinite(dj) && dj > 0) {
+                            e = fabs(z - dj) / dj / P.geo_tol;
+                            if (e > 1)
+                                e = 1;
+                        }
+                    }
+                }
+                geo += e;
+            }
+        }
+        if (ng)
+            geo /= ng;
+    }
+    return (1.0 - ph) + P.geo_w * geo;
+}
+
+__device__ __forceinline__ bool colour_pixel(const PMK &P, int64_t t, int &y, int &x)
+{
+    const int W2 = (P.W + 1) / 2;
+    y = (int)(t / W2);
+    const int j = (int)(t - (int64_t)y * W2);
+    x = 2 * j + ((P.colour + y) & 1);
+    return y < P.H && x < P.W;
+}
+
+__device__ __forceinline__ void count_changes(const PMK &P, int n)
+{
+    if (n)
+        atomicAdd(P.changed, (unsigned long long)n);
+}
+
+// A: current plane + propagation from the 8 neighbours (:614-645)
+template <int SM>
+__global__ void __launch_bounds__(128) pm_propagate_kernel(const PMK P)
+{
+    static const int oy[8] = {-1, 1, 0, 0, -5, 5, 0, 0};
+    static const int ox[8] = {0, 0, -1, 1, 0, 0, -5, 5};
+    int y, x;
+    if (!colour_pixel(P, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, y, x))
+        return;
+    const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
+    if (!P.active[p])
+        return;
+    double photo = -1.0;
+    double n0 = P.normal[p], n1 = P.normal[HW + p], n2 = P.normal[2 * HW + p];
+    double best, sc;
+    const float d0 = P.depth[p];
+    if (isfinite(d0)) {
+        best = pm_cost<SM>(P, y, x, d0, n0, n1, n2, photo);
+        sc = photo;
+    } else {
+        best = 1e9;
+        sc = -1;
+    }
+    int changed = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int yn = y + oy[k], xn = x + ox[k];
+        if (yn < 0 || xn < 0 || yn >= P.H || xn >= P.W)
+            continue;
+        const size_t q = (size_t)yn * P.W + xn;
+        const float dq = P.depth[q];
+        if (!isfinite(dq))
+            continue;
+        const double b0 = P.normal[q], b1 = P.normal[HW + q], b2 = P.normal[2 * HW + q];
+        const double dn = b0 * P.ray[q] + b1 * P.ray[HW + q] + b2 * P.ray[2 * HW + q];
+        const double ndr = b0 * P.ray[p] + b1 * P.ray[HW + p] + b2 * P.ray[2 * HW + p];
+        if (ndr >= -1e-6)
+            continue;
+        const double dp = (double)dq * dn / ndr;
+        if (dp < P.dmin || dp > P.dmax)
+            continue;
+        const double cst = pm_cost<SM>(P, y, x, dp, b0, b1, b2, photo);
+        if (cst < best) {
+            best = cst;
+            sc = photo;
+            P.depth[p] = (float)dp;
+            P.normal[p] = (float)b0;
+            P.normal[HW + p] = (float)b1;
+            P.normal[2 * HW + p] = (float)b2;
+            changed += 1;
+        }
+    }
+    P.best[p] = best;
+    P.sc[p] = sc;
+    count_changes(P, changed);
+}
+
+// B1: per row, the rank of every drawing pixel (active, this colour, finite depth) in x order
+__global__ void __launch_bounds__(256) pm_rank_rows_kernel(const PMK P)
+{
+    __shared__ int wsum[4];
+    __shared__ int carry_s;
+    const int y = blockIdx.x;
+    const int W2 = (P.W + 1) / 2;
+    const int x0 = (P.colour + y) & 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0)
+        carry_s = 0;
+    __syncthreads();
+    for (int j0 = 0; j0 < W2; j0 += 256) {
+        const int j = j0 + threadIdx.x;
+        const int x = 2 * j + x0;
+        const size_t p = (size_t)y * P.W + x;
+        const bool flag = j < W2 && x < P.W && P.active[p] && isfinite(P.depth[p]);
+        const unsigned long long m = __ballot(flag);
+        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0)
+            wsum[wv] = __popcll(m);
+        __syncthreads();
+        int off = carry_s;
+        for (int w = 0; w < wv; ++w)
+            off += wsum[w];
+        if (j < W2 && x < P.W)
+            P.rank[p] = flag ? off + pre : -1;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            carry_s += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        P.row_cnt[y] = carry_s;
+}
+
+// B2: exclusive scan of the row counts; stream state after the pass = jump(state, 8 * total)
+__global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P)
+{
+    __shared__ long long part[256];
+    const int H = P.H;
+    const int per = (H + 255) / 256;
+    const int a = threadIdx.x * per, b = min(H, a + per);
+    long long s = 0;
+    for (int y = a; y < b; ++y)
+        s += P.row_cnt[y];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = 0;
+        for (int t = 0; t < 256; ++t) {
+            const long long v = part[t];
+            part[t] = acc;
+            acc += v;
+        }
+        *P.st_out = xs_jump(*P.st_in, 8ull * (unsigned long long)acc);
+    }
+    __syncthreads();
+    long long off = part[threadIdx.x];
+    for (int y = a; y < b; ++y) {
+        P.row_off[y] = (int)off;
+        off += P.row_cnt[y];
+    }
+}
+
+// C: the 2 random refinements with the pixel's own slice of the stream (:646-664); score write
+template <int SM>
+__global__ void __launch_bounds__(128) pm_random_kernel(const PMK P)
+{
+    int y, x;
+    if (!colour_pixel(P, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, y, x))
+        return;
+    const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
+    if (!P.active[p])
+        return;
+    double best = P.best[p], sc = P.sc[p];
+    const int rk = P.rank[p];
+    int changed = 0;
+    if (rk >= 0) {
+        uint32_t st = xs_jump(*P.st_in, 8ull * (unsigned long long)(P.row_off[y] + rk));
+        double photo = -1.0;
+        for (int tr = 0; tr < 2; ++tr) {
+            const double dp = (double)P.depth[p] * (1 + P.dsc * (2 * xs_rnd(st) - 1));
+            double b0 = (double)P.normal[p] + P.nsc * (2 * xs_rnd(st) - 1);
+            double b1 = (double)P.normal[HW + p] + P.nsc * (2 * xs_rnd(st) - 1);
+            double b2 = (double)P.normal[2 * HW + p] + P.nsc * (2 * xs_rnd(st) - 1);
+            const double nn = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+            if (nn < 1e-9 || dp < P.dmin || dp > P.dmax)
+                continue;
+            b0 /= nn;
+            b1 /= nn;
+            b2 /= nn;
+            const double cst = pm_cost<SM>(P, y, x, dp, b0, b1, b2, photo);
+            if (cst < best) {
+                best = cst;
+                sc = photo;
+                P.depth[p] = (float)dp;
+                P.normal[p] = (float)b0;
+                P.normal[HW + p] = (float)b1;
+                P.normal[2 * HW + p] = (float)b2;
+                changed += 1;
+            }
+        }
+    }
+    P.score[p] = (float)sc;
+    count_changes(P, changed);
+}
+
+__global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed)
+{
+    st[0] = seed | 1u;
+    *changed = 0ull;
+}
+
+// workspace layout
+struct PMWs {
+    double *best, *sc;
+    int *rank, *row_cnt, *row_off;
+    uint32_t *st;  // 2 states
+    unsigned long long *changed;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static size_t pm_ws_layout(int H, int W, char *base, PMWs *w)
+{
+    const size_t HW = (size_t)H * W;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + o : nullptr;
+        o += align256(bytes);
+        return p;
+    };
+    char *b = take(HW * 8), *s = take(HW * 8), *rk = take(HW * 4), *rc = take((size_t)H * 4),
+         *ro = take((size_t)H * 4), *st = take(16), *ch = take(8);
+    if (w) {
+        w->best = (double *)b;
+        w->sc = (double *)s;
+        w->rank = (int *)rk;
+        w->row_cnt = (int *)rc;
+        w->row_off = (int *)ro;
+        w->st = (uint32_t *)st;
+        w->changed = (unsigned long long *)ch;
+    }
+    return o;
+}
+
+template <int SM>
+static int pm_run(PMK &P, int iters, hipStream_t stream, uint32_t seed, const PMWs &w)
+{
+    const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
+    const unsigned gA = (unsigned)cdiv(ncol, 128);
+    hipLaunchKernelGGL(pm_init_kernel, dim3(1), dim3(1), 0, stream, w.st, seed, w.changed);
+    int pass = 0;
+    for (int it = 0; it < iters; ++it) {
+        P.dsc = 0.05 / (it + 1);
+        P.nsc = 0.3 / (it + 1);
+        for (int colour = 0; colour < 2; ++colour, ++pass) {
+            P.colour = colour;
+            P.st_in = w.st + (pass & 1);
+            P.st_out = w.st + ((pass + 1) & 1);
+            {
+                ProfScope ps("pm_propagate", stream);
+                hipLaunchKernelGGL(pm_propagate_kernel<SM>, dim3(gA), dim3(128), 0, stream, P);
+            }
+            {
+                ProfScope ps("pm_rank_rows", stream);
+                hipLaunchKernelGGL(pm_rank_rows_kernel, dim3((unsigned)P.H), dim3(256), 0, stream, P);
+            }
+            {
+                ProfScope ps("pm_scan_rows", stream);
+                hipLaunchKernelGGL(pm_scan_rows_kernel, dim3(1), dim3(256), 0, stream, P);
+            }
+            {
+                ProfScope ps("pm_random", stream);
+                hipLaunchKernelGGL(pm_random_kernel<SM>, dim3(gA), dim3(128), 0, stream, P);
+            }
+        }
+    }
+    return launch_check("pm kernels");
+}
+
+}  // namespace om
 
 using namespace om;
 
-extern "C" size_t om_pm_workspace_bytes(int32_t H, int32_t W) { return (size_t)(H > 0 ? H : 0) * (W > 0 ? W : 0) * 8; }
-
-extern "C" int om_pm_refine(const float *, int32_t, int32_t, const float *, const double *, int32_t,
-                            const float *const *, const float *const *, const int32_t *, const int32_t *,
-                            const double *, float *, float *, float *, const uint8_t *, int32_t, int32_t, int32_t,
-                            double, double, int32_t, double, double, uint32_t, void *, size_t, int64_t *, void *)
+extern "C" size_t om_pm_workspace_bytes(int32_t H, int32_t W)
 {
-    return fail(OM_EUNSUP, "pm_refine not built yet");
+    if (H <= 0 || W <= 0)
+        return 256;
+    return pm_ws_layout(H, W, nullptr, nullptr);
+}
+
+extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
+                            const float *const *imgs, const float *const *deps, const int32_t *ih, const int32_t *iw,
+                            const double *par, float *depth, float *normal, float *score, const uint8_t *active,
+                            int32_t r, int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters,
+                            double dmin, double dmax, uint32_t seed, void *workspace, size_t workspace_bytes,
+                            int64_t *changed, void *stream)
+{
+    if (changed)
+        *changed = 0;
+    if (S <= 0 || S > 16)  // the reference returns 0 without work (_dense.pyx:587-588)
+        return OM_OK;
+    if (H < 0 || W < 0)
+        return fail(OM_ESHAPE, "negative size");
+    if (step <= 0)
+        return fail(OM_EINVAL, "pm_refine needs step >= 1");
+    if (top_k < 1)
+        return fail(OM_EINVAL, "pm_refine needs top_k >= 1");
+    if (!imgs || !deps || !ih || !iw || !par || !Cr)
+        return fail(OM_EINVAL, "null host array");
+    if ((int64_t)H * W == 0 || iters <= 0)
+        return OM_OK;
+    const size_t need = pm_ws_layout(H, W, nullptr, nullptr);
+    if (!workspace || workspace_bytes < need)
+        return fail(OM_EINVAL, "workspace of %zu bytes needed (got %zu)", need, workspace_bytes);
+    PMWs w;
+    pm_ws_layout(H, W, (char *)workspace, &w);
+    PMK P;
+    memset(&P, 0, sizeof P);
+    P.ref = ref;
+    P.ray = ray;
+    P.active = active;
+    P.depth = depth;
+    P.normal = normal;
+    P.score = score;
+    P.H = H;
+    P.W = W;
+    P.S = S;
+    P.r = r;
+    P.step = step;
+    P.kk = top_k < S ? top_k : S;
+    P.cr0 = Cr[0];
+    P.cr1 = Cr[1];
+    P.cr2 = Cr[2];
+    P.geo_w = geo_w;
+    P.geo_tol = geo_tol;
+    P.dmin = dmin;
+    P.dmax = dmax;
+    P.best = w.best;
+    P.sc = w.sc;
+    P.rank = w.rank;
+    P.row_cnt = w.row_cnt;
+    P.row_off = w.row_off;
+    P.changed = w.changed;
+    P.pinhole = 1;
+    for (int s = 0; s < S; ++s) {
+        if (ih[s] < 1 || iw[s] < 1)
+            return fail(OM_ESHAPE, "source %d has an empty image", s);
+        P.img[s] = imgs[s];
+        P.dep[s] = deps[s];
+        P.ih[s] = ih[s];
+        P.iw[s] = iw[s];
+        for (int t = 0; t < 18; ++t)
+            P.par[s][t] = par[18 * s + t];
+        if (par[18 * s + 13] != 0.0 || par[18 * s + 14] != 0.0 || par[18 * s + 15] != 0.0)
+            P.pinhole = 0;
+    }
+    hipStream_t st = as_stream(stream);
+    int rc;
+    if (S <= 4)
+        rc = pm_run<4>(P, iters, st, seed, w);
+    else if (S <= 8)
+        rc = pm_run<8>(P, iters, st, seed, w);
+    else if (S <= 12)
+        rc = pm_run<12>(P, iters, st, seed, w);
+    else
+        rc = pm_run<16>(P, iters, st, seed, w);
+    if (rc)
+        return rc;
+    if (changed) {
+        unsigned long long h = 0;
+        if ((rc = hip_check(hipMemcpyAsync(&h, w.changed, sizeof h, hipMemcpyDeviceToHost, st), "copy changed")))
+            return rc;
+        if ((rc = hip_check(hipStreamSynchronize(st), "pm_refine")))
+            return rc;
+        *changed = (int64_t)h;
+    }
+    return OM_OK;
 }

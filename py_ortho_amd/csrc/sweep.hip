@@ -548,6 +548,7 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
     const size_t lds = sizeof(float) * (2 * halo + 4 * (size_t)(Pl.TH + 2 * r) * kTW);
     const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
     hipStream_t st = as_stream(stream);
+    ProfScope ps("sweep", st);
     switch (Pl.k) {
     case 1: hipLaunchKernelGGL(sweep_kernel<1>, grid, dim3(kThreads), lds, st, Pl); break;
     case 2: hipLaunchKernelGGL(sweep_kernel<2>, grid, dim3(kThreads), lds, st, Pl); break;
@@ -558,6 +559,7 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
         return rc;
     if (Pl.chunks > 1) {
         const int64_t HW = (int64_t)Pl.H * Pl.W;
+        ProfScope pm("sweep_merge", st);
         hipLaunchKernelGGL(sweep_merge_kernel, dim3((unsigned)cdiv(HW, 256)), dim3(256), 0, st, Pl.part, Pl.chunks,
                            HW, Pl.best, Pl.prev, Pl.spb, Pl.snb, Pl.idx, Pl.i0, Pl.hc);
         if ((rc = launch_check("sweep_merge_kernel")))

@@ -261,6 +261,17 @@ def half(g) -> torch.Tensor:
     return out
 
 
+def rays_world(H: int, W: int, R, cx: float, cy: float, f: float, k1: float = 0.0, k2: float = 0.0,
+               k3: float = 0.0, device="cuda") -> torch.Tensor:
+    """densify._rays_world (densify.py:288-297): (3, H, W) float32 world rays (camera-axis depth 1)."""
+    Rm = np.ascontiguousarray(np.asarray(R, np.float64).reshape(3, 3))
+    out = torch.empty((3, H, W), dtype=torch.float32, device=device)
+    rd = (ctypes.c_double * 9)(*Rm.ravel().tolist())
+    check(_lib.lib().om_rays_world(int(H), int(W), rd, float(cx), float(cy), float(f), float(k1), float(k2),
+                                   float(k3), out.data_ptr(), _stream()), "rays_world")
+    return out
+
+
 def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active, r, step, top_k, geo_w, geo_tol,
               iters, dmin, dmax, seed) -> int:
     """PatchMatch repair of a depth map (_dense.pyx:566-666): only `active` pixels are re-estimated;

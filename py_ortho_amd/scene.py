@@ -1,0 +1,228 @@
+"""Seeded synthetic survey scenes with exact per-view depth, for tests and benchmarks.
+
+Terrain: a base plane at 0 m plus Gaussian bumps (heights summed, clipped to [0, hmax]).
+Texture: multi-octave value noise on the ground plane (hashed lattice, smoothstep interpolation),
+normalised to [0, 1], plus Gaussian pixel noise.  Views: pinhole cameras (pyOrthomosaic
+convention: pixel centres at integers, cx = (W-1)/2, R world->camera, camera-axis depth), rendered
+by ray casting the height field (fixed-step march to the first crossing, then bisection).
+
+Patterns from the reference's synthetic survey (tests/synthetic3d.py:106-151, tests/synthetic.py:24-53),
+written for torch so that drone-size views render on the device.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+NADIR = np.diag([1.0, -1.0, -1.0])  # image x = east, image y = south, looking down
+
+
+@dataclass
+class Camera:
+    R: np.ndarray       # (3, 3) world -> camera
+    C: np.ndarray       # (3,) centre
+    f: float
+    cx: float
+    cy: float
+    w: int
+    h: int
+    k1: float = 0.0
+    k2: float = 0.0
+    k3: float = 0.0
+
+    def par18(self) -> np.ndarray:
+        """PatchMatch source layout (_dense.pyx:430): R (9), C (3), f, k1, k2, k3, cx, cy."""
+        return np.r_[self.R.ravel(), self.C, [self.f, self.k1, self.k2, self.k3, self.cx, self.cy]]
+
+    def geo18(self, ref: "Camera") -> np.ndarray:
+        """Sweep source layout (_dense.pyx:748): Rs (9), b = Rs (C_ref - C_src) (3), f, k1, k2, k3, cx, cy."""
+        b = self.R @ (ref.C - self.C)
+        return np.r_[self.R.ravel(), b, [self.f, self.k1, self.k2, self.k3, self.cx, self.cy]]
+
+
+def pinhole(C, f, w, h, R=NADIR) -> Camera:
+    return Camera(np.asarray(R, np.float64), np.asarray(C, np.float64), float(f), (w - 1) / 2.0, (h - 1) / 2.0,
+                  int(w), int(h))
+
+
+@dataclass
+class Terrain:
+    cx: np.ndarray      # bump centres x (m)
+    cy: np.ndarray
+    height: np.ndarray  # bump heights (m)
+    sigma: np.ndarray   # bump widths (m)
+    hmax: float
+    tex_seed: int
+    octaves: Tuple[Tuple[float, float], ...] = ((8.0, 0.5), (2.0, 0.25), (0.5, 0.15), (0.13, 0.1))
+
+    def height_at(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+        dev = X.device
+        z = torch.zeros_like(X)
+        for k in range(len(self.height)):
+            d2 = (X - float(self.cx[k])) ** 2 + (Y - float(self.cy[k])) ** 2
+            z = z + float(self.height[k]) * torch.exp(-d2 / (2.0 * float(self.sigma[k]) ** 2))
+        del dev
+        return torch.clamp(z, 0.0, self.hmax)
+
+
+def make_terrain(seed: int, extent: Tuple[float, float, float, float], n_bumps: int = 20, hmax: float = 15.0,
+                 sigma: Tuple[float, float] = (5.0, 20.0), tex_seed: Optional[int] = None) -> Terrain:
+    """extent = (x0, x1, y0, y1) in metres over which the bump centres are drawn."""
+    rng = np.random.default_rng(seed)
+    x0, x1, y0, y1 = extent
+    return Terrain(rng.uniform(x0, x1, n_bumps), rng.uniform(y0, y1, n_bumps), rng.uniform(0.0, hmax, n_bumps),
+                   rng.uniform(sigma[0], sigma[1], n_bumps), hmax, seed + 1 if tex_seed is None else tex_seed)
+
+
+def _hash01(i: torch.Tensor, j: torch.Tensor, salt: int) -> torch.Tensor:
+    """Deterministic lattice value in [0, 1) from integer coordinates (same on CPU and device)."""
+    M = 0xFFFFFFFF
+    h = (i * 374761393 + j * 668265263 + salt * 2246822519) & M
+    h = ((h ^ (h >> 13)) * 1274126177) & M
+    h = h ^ (h >> 16)
+    return (h & 0xFFFFFF).to(torch.float64) / 16777216.0
+
+
+def texture(terr: Terrain, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+    """Multi-octave value noise at ground positions, normalised by the total amplitude to [0, 1]."""
+    out = torch.zeros_like(X)
+    tot = 0.0
+    for o, (wl, amp) in enumerate(terr.octaves):
+        gx, gy = X / wl, Y / wl
+        fx, fy = torch.floor(gx), torch.floor(gy)
+        tx, ty = gx - fx, gy - fy
+        sx, sy = tx * tx * (3 - 2 * tx), ty * ty * (3 - 2 * ty)
+        i, j = fx.to(torch.int64), fy.to(torch.int64)
+        salt = terr.tex_seed * 131 + o
+        v00, v10 = _hash01(i, j, salt), _hash01(i + 1, j, salt)
+        v01, v11 = _hash01(i, j + 1, salt), _hash01(i + 1, j + 1, salt)
+        out = out + amp * ((v00 * (1 - sx) + v10 * sx) * (1 - sy) + (v01 * (1 - sx) + v11 * sx) * sy)
+        tot += amp
+    return out / tot
+
+
+def camera_rays(cam: Camera, device, dtype=torch.float64) -> torch.Tensor:
+    """(3, h, w) world directions with camera-axis depth 1 (pinhole; distortion-free cameras)."""
+    v, u = torch.meshgrid(torch.arange(cam.h, device=device, dtype=dtype),
+                          torch.arange(cam.w, device=device, dtype=dtype), indexing="ij")
+    xc = torch.stack([(u - cam.cx) / cam.f, (v - cam.cy) / cam.f, torch.ones_like(u)])
+    R = torch.as_tensor(cam.R, dtype=dtype, device=device)
+    return torch.einsum("ji,jhw->ihw", R, xc)
+
+
+def render(terr: Terrain, cam: Camera, noise_seed: int, noise: float = 0.01, device="cpu", march: int = 96,
+           bisect: int = 40, rows_per_chunk: int = 256):
+    """-> (image (h, w) float32 in [0,1] + noise, depth (h, w) float32 camera-axis)."""
+    img = torch.empty((cam.h, cam.w), dtype=torch.float32, device=device)
+    dep = torch.empty((cam.h, cam.w), dtype=torch.float32, device=device)
+    C = [float(c) for c in cam.C]
+    full = camera_rays(cam, device)
+    for r0 in range(0, cam.h, rows_per_chunk):
+        d = full[:, r0:r0 + rows_per_chunk]
+        dz = d[2]
+        # bracket: from the plane z = hmax (above all terrain) to z = 0 (below or on all terrain)
+        t_top = torch.clamp((C[2] - terr.hmax) / (-dz), min=0.0)
+        t_bot = C[2] / (-dz)
+        lo, hi = t_top.clone(), t_bot.clone()
+        found = torch.zeros_like(lo, dtype=torch.bool)
+        prev_t = t_top
+        for s in range(1, march + 1):
+            t = t_top + (t_bot - t_top) * (s / march)
+            X, Y, Z = C[0] + t * d[0], C[1] + t * d[1], C[2] + t * dz
+            below = Z <= terr.height_at(X, Y)
+            hit = below & ~found
+            lo = torch.where(hit, prev_t, lo)
+            hi = torch.where(hit, t, hi)
+            found = found | below
+            prev_t = t
+        for _ in range(bisect):
+            m = 0.5 * (lo + hi)
+            X, Y, Z = C[0] + m * d[0], C[1] + m * d[1], C[2] + m * dz
+            below = Z <= terr.height_at(X, Y)
+            hi = torch.where(below, m, hi)
+            lo = torch.where(below, lo, m)
+        t = 0.5 * (lo + hi)
+        X, Y = C[0] + t * d[0], C[1] + t * d[1]
+        img[r0:r0 + rows_per_chunk] = texture(terr, X, Y).to(torch.float32)
+        dep[r0:r0 + rows_per_chunk] = t.to(torch.float32)
+    g = torch.Generator(device="cpu").manual_seed(int(noise_seed))
+    if noise > 0:
+        img += (torch.randn((cam.h, cam.w), generator=g, dtype=torch.float32) * noise).to(device)
+    return img, dep
+
+
+@dataclass
+class View:
+    cam: Camera
+    img: torch.Tensor
+    depth: torch.Tensor
+
+
+@dataclass
+class Scene:
+    terrain: Terrain
+    ref: View
+    srcs: List[View] = field(default_factory=list)
+
+    def geo(self) -> np.ndarray:
+        return np.stack([v.cam.geo18(self.ref.cam) for v in self.srcs])
+
+    def par(self) -> np.ndarray:
+        return np.stack([v.cam.par18() for v in self.srcs])
+
+
+def make_scene(seed: int, W: int, H: int, f: float, alt: float, src_xy: Sequence[Tuple[float, float]],
+               device="cpu", noise: float = 0.01, ref_xy=(0.0, 0.0)) -> Scene:
+    """Reference camera at (ref_xy, alt) and one source camera per (x, y), all nadir."""
+    half_w = (W / 2) / f * alt
+    half_h = (H / 2) / f * alt
+    xs = [p[0] for p in src_xy] + [ref_xy[0]]
+    ys = [p[1] for p in src_xy] + [ref_xy[1]]
+    ext = (min(xs) - half_w, max(xs) + half_w, min(ys) - half_h, max(ys) + half_h)
+    terr = make_terrain(seed, ext)
+    cams = [pinhole((ref_xy[0], ref_xy[1], alt), f, W, H)] + [pinhole((x, y, alt), f, W, H) for x, y in src_xy]
+    views = []
+    for k, cam in enumerate(cams):
+        img, dep = render(terr, cam, noise_seed=seed * 1000 + 2 + k, noise=noise, device=device)
+        views.append(View(cam, img, dep))
+    return Scene(terr, views[0], views[1:])
+
+
+def config_c1(device="cpu", scale: float = 1.0) -> Scene:
+    """Plain plane sweep: 640x480, f=600, 80 m, 4 sources on a 15 m cross (seed 0)."""
+    W, H = int(round(640 * scale)), int(round(480 * scale))
+    b = 15.0
+    return make_scene(0, W, H, 600.0 * scale, 80.0, [(b, 0.0), (-b, 0.0), (0.0, b), (0.0, -b)], device=device)
+
+
+def config_c2(device="cpu", scale: float = 1.0) -> Scene:
+    """Single-view PatchMatch: 2000x1500, f=1800, 100 m, 8 sources on a 2x4 grid, 20 m baseline."""
+    W, H = int(round(2000 * scale)), int(round(1500 * scale))
+    src = [(x, y) for y in (-10.0, 10.0) for x in (-30.0, -10.0, 10.0, 30.0)]
+    return make_scene(3, W, H, 1800.0 * scale, 100.0, src, device=device)
+
+
+def pm_init(seed: int, H: int, W: int, ray: torch.Tensor, dmin: float, dmax: float, max_tilt_deg: float = 30.0):
+    """Seeded random slanted planes: depth U(dmin, dmax), normals uniform in a cone of
+    max_tilt_deg around up, flipped to face the camera.  -> (depth (H,W), normal (3,H,W)) float32."""
+    rng = np.random.default_rng(seed)
+    depth = rng.uniform(dmin, dmax, (H, W)).astype(np.float32)
+    cmin = math.cos(math.radians(max_tilt_deg))
+    cz = rng.uniform(cmin, 1.0, (H, W))
+    phi = rng.uniform(0, 2 * math.pi, (H, W))
+    sz = np.sqrt(1 - cz * cz)
+    n = np.stack([sz * np.cos(phi), sz * np.sin(phi), cz])
+    r = ray.detach().cpu().numpy().astype(np.float64)
+    flip = (n * r).sum(0) > 0
+    n[:, flip] *= -1
+    return torch.from_numpy(depth), torch.from_numpy(np.ascontiguousarray(n, np.float32))
+
+
+def rays_world(cam: Camera, device="cpu") -> torch.Tensor:
+    """(3, h, w) float32 rays of the reference camera as densify._rays_world builds them for a
+    pinhole camera (densify.py:288-297): x = (u - cx) / f, y = (v - cy) / f, then R^T."""
+    return camera_rays(cam, device).to(torch.float32).contiguous()
