@@ -44,7 +44,8 @@ constexpr int kRunsPerRow = kTW / kRun;           // 8
 constexpr int kHRows = kThreads / kRunsPerRow;    // 32 halo rows per horizontal pass
 constexpr int kVSeg = kThreads / kTW;             // 4 row segments per column in the vertical pass
 constexpr int kMaxR = 7;                          // largest window radius of the fused kernel
-constexpr int kMaxSlots = 10;                     // halo pixels cached per thread: 10*256 >= 32*78
+constexpr int kVR = 4;                            // output rows per thread in the vertical pass
+constexpr int kTH = kVSeg * kVR;                  // 16 output rows per tile
 constexpr int kPartPlanes = 6;                    // chunk partials: best, idx, prev-of-best, next-of-best, first, last
 
 struct SrcK {
@@ -105,14 +106,17 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
 template <int K>
 __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
 {
-    extern __shared__ float lds[];
-    const int r = P.r, TH = P.TH;
+    extern __shared__ double lds_d[];
+    const int r = P.r;
+    constexpr int TH = kTH;
     const int HR = TH + 2 * r;          // halo rows
     const int HC = kTW + 2 * r;         // halo columns
     const int NHALO = HR * HC;
-    float *sI = lds;                    // (HR, HC) reference
-    float *sJ = sI + NHALO;             // (HR, HC) warped source, NaN = invalid
-    float *sH = sJ + NHALO;             // (4, HR, kTW) horizontal sums
+    double *sD = lds_d;                          // (HR, HC) depth of the current hypothesis, NaN outside
+    int *sP = (int *)(sD + NHALO);               // (HR, HC) pixel index, -1 outside the image
+    float *sI = (float *)(sP + NHALO);           // (HR, HC) reference
+    float *sJ = sI + NHALO;                      // (HR, HC) warped source, NaN = invalid
+    float *sH = sJ + NHALO;                      // (4, HR, kTW) horizontal sums
 
     const int tid = threadIdx.x;
     const int ntiles = P.tiles_x * P.tiles_y;
@@ -124,45 +128,30 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
     const int ib = min(P.i0 + P.nh, ia + P.hc);
     const int H = P.H, W = P.W;
     const size_t HW = (size_t)H * W;
+    const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
 
-    // ---- halo pixels owned by this thread in the warp pass: ray and generator inputs cached
-    bool hin[kMaxSlots];
-    int hpix[kMaxSlots];
-    float hrx[kMaxSlots], hry[kMaxSlots], hrz[kMaxSlots], hga[kMaxSlots], hgb[kMaxSlots];
-    double hd[kMaxSlots];
-#pragma unroll
-    for (int s = 0; s < kMaxSlots; ++s) {
-        const int hp = tid + s * kThreads;
+    // ---- the tile + halo: pixel index and reference value
+    for (int hp = tid; hp < NHALO; hp += kThreads) {
         const int hy = hp / HC, hx = hp - hy * HC;
         const int gy = ty0 - r + hy, gx = tx0 - r + hx;
-        const bool in = hp < NHALO && gy >= 0 && gy < H && gx >= 0 && gx < W;
-        hin[s] = in;
-        const int p = in ? gy * W + gx : 0;
-        hpix[s] = p;
-        hrx[s] = in ? P.ray[p] : 0.f;
-        hry[s] = in ? P.ray[HW + p] : 0.f;
-        hrz[s] = in ? P.ray[2 * HW + p] : 0.f;
-        hga[s] = (in && (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE)) ? P.ha[p] : 0.f;
-        hgb[s] = (in && (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE)) ? P.hb[p] : 0.f;
-        if (hp < NHALO)
-            sI[hp] = in ? P.ref[p] : 0.f;
-        hd[s] = 0.0;
+        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const int p = in ? gy * W + gx : -1;
+        sP[hp] = p;
+        sI[hp] = in ? P.ref[p] : 0.f;
     }
 
-    // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+VR-1
-    constexpr int kMaxVR = 8;  // TH <= 32 -> VR <= 8
-    const int VR = TH / kVSeg;
-    const int vc = tid % kTW, vseg = tid / kTW, vy0 = vseg * VR;
+    // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+kVR-1
+    const int vc = tid % kTW, vseg = tid / kTW, vy0 = vseg * kVR;
     const int ox = tx0 + vc;
-    float mu_[kMaxVR], sd_[kMaxVR];
-    float st_best[kMaxVR], st_prev[kMaxVR], st_spb[kMaxVR], st_snb[kMaxVR], st_first[kMaxVR];
-    int st_idx[kMaxVR];
-    float tk[kMaxVR][K];
+    float mu_[kVR], sd_[kVR];
+    float st_best[kVR], st_prev[kVR], st_spb[kVR], st_snb[kVR], st_first[kVR];
+    int st_idx[kVR];
+    float tk[kVR][K];
     const bool chunked = P.chunks > 1;
 #pragma unroll
-    for (int j = 0; j < kMaxVR; ++j) {
+    for (int j = 0; j < kVR; ++j) {
         const int oy = ty0 + vy0 + j;
-        const bool in = j < VR && oy < H && ox < W;
+        const bool in = oy < H && ox < W;
         const size_t p = in ? (size_t)oy * W + ox : 0;
         mu_[j] = in ? P.mu[p] : 0.f;
         sd_[j] = in ? P.sd[p] : 1.f;
@@ -182,21 +171,26 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
         st_first[j] = NAN;
     }
     const double kk = 1.0 / (double)((2 * r + 1) * (2 * r + 1));
+    __syncthreads();
 
     for (int i = ia; i < ib; ++i) {
-        // inverse depth -> depth per cached halo pixel (shared by all sources)
-#pragma unroll
-        for (int s = 0; s < kMaxSlots; ++s) {
-            if (hin[s]) {
-                const float inv = hyp_value(P, i, hpix[s], hga[s], hgb[s]);
-                hd[s] = 1.0 / (double)inv;
+        // inverse depth -> depth per halo pixel (shared by all sources); the previous
+        // hypothesis's last warp pass finished reading sD before the barrier after it
+        for (int hp = tid; hp < NHALO; hp += kThreads) {
+            const int p = sP[hp];
+            double d = NAN;
+            if (p >= 0) {
+                const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
+                d = 1.0 / (double)hyp_value(P, i, p, a, b);
             }
+            sD[hp] = d;
         }
 #pragma unroll
-        for (int j = 0; j < kMaxVR; ++j)
+        for (int j = 0; j < kVR; ++j)
 #pragma unroll
             for (int a = 0; a < K; ++a)
                 tk[j][a] = -INFINITY;
+        __syncthreads();
 
         for (int s = 0; s < P.n; ++s) {
             const SrcK &S = P.src[s];
@@ -205,41 +199,38 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
             const float *img = S.img;
             const int sw = S.w;
             // ---- W: warp the halo into LDS (_sweep.h:49-80)
-#pragma unroll
-            for (int q = 0; q < kMaxSlots; ++q) {
-                const int hp = tid + q * kThreads;
-                if (hp < NHALO) {
-                    float J = NAN;
-                    if (hin[q]) {
-                        const double d = hd[q];
-                        const double rx = hrx[q], ry = hry[q], rz = hrz[q];
-                        const double X = g[9] + d * ((g[0] * rx + g[1] * ry) + g[2] * rz);
-                        const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
-                        const double Z = g[11] + d * ((g[6] * rx + g[7] * ry) + g[8] * rz);
-                        if (!(Z <= 1e-6)) {
-                            const double nx = X / Z, ny = Y / Z;
-                            double dd;
-                            if (P.pinhole) {
-                                dd = g[12];  // f * (1 + r2 * 0) == f for finite r2
-                            } else {
-                                const double rr = nx * nx + ny * ny;
-                                dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
-                            }
-                            const double u = dd * nx + g[16], v = dd * ny + g[17];
-                            if (u >= 0.0 && v >= 0.0 && u <= wl && v <= hl) {
-                                const int x0 = (int)u, y0 = (int)v;
-                                const double fx = u - (double)x0, fy = v - (double)y0;
-                                const float *pp = img + (size_t)y0 * sw + x0;
-                                const double a00 = pp[0], a01 = pp[1], a10 = pp[sw], a11 = pp[sw + 1];
-                                const double val =
-                                    (a00 * (1.0 - fx) + a01 * fx) * (1.0 - fy) + (a10 * (1.0 - fx) + a11 * fx) * fy;
-                                if (isfinite(val))
-                                    J = (float)val;
-                            }
+            for (int hp = tid; hp < NHALO; hp += kThreads) {
+                float J = NAN;
+                const int p = sP[hp];
+                if (p >= 0) {
+                    const double d = sD[hp];
+                    const double rx = P.ray[p], ry = P.ray[HW + p], rz = P.ray[2 * HW + p];
+                    const double X = g[9] + d * ((g[0] * rx + g[1] * ry) + g[2] * rz);
+                    const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
+                    const double Z = g[11] + d * ((g[6] * rx + g[7] * ry) + g[8] * rz);
+                    if (!(Z <= 1e-6)) {
+                        const double nx = X / Z, ny = Y / Z;
+                        double dd;
+                        if (P.pinhole) {
+                            dd = g[12];  // f * (1 + r2 * 0) == f for finite r2
+                        } else {
+                            const double rr = nx * nx + ny * ny;
+                            dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
+                        }
+                        const double u = dd * nx + g[16], v = dd * ny + g[17];
+                        if (u >= 0.0 && v >= 0.0 && u <= wl && v <= hl) {
+                            const int x0 = (int)u, y0 = (int)v;
+                            const double fx = u - (double)x0, fy = v - (double)y0;
+                            const float *pp = img + (size_t)y0 * sw + x0;
+                            const double a00 = pp[0], a01 = pp[1], a10 = pp[sw], a11 = pp[sw + 1];
+                            const double val =
+                                (a00 * (1.0 - fx) + a01 * fx) * (1.0 - fy) + (a10 * (1.0 - fx) + a11 * fx) * fy;
+                            if (isfinite(val))
+                                J = (float)val;
                         }
                     }
-                    sJ[hp] = J;
                 }
+                sJ[hp] = J;
             }
             __syncthreads();
             // ---- H: horizontal window sums of one 8-column run per thread (_sweep.h:85-114)
@@ -293,8 +284,8 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
                     c3 = c3 + (double)h3[o];
                 }
 #pragma unroll
-                for (int j = 0; j < kMaxVR; ++j) {
-                    if (j < VR) {
+                for (int j = 0; j < kVR; ++j) {
+                    {
                         if (j > 0) {
                             const int on = (vy0 + j + 2 * r) * kTW, oo = (vy0 + j - 1) * kTW;
                             double t;
@@ -329,8 +320,8 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
         }
         // ---- combine + streaming winner (_sweep.h:181-214)
 #pragma unroll
-        for (int j = 0; j < kMaxVR; ++j) {
-            if (j < VR) {
+        for (int j = 0; j < kVR; ++j) {
+            {
                 float s = 0.0f;
 #pragma unroll
                 for (int a = 0; a < K; ++a)
@@ -352,9 +343,9 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
     }
 
 #pragma unroll
-    for (int j = 0; j < kMaxVR; ++j) {
+    for (int j = 0; j < kVR; ++j) {
         const int oy = ty0 + vy0 + j;
-        if (j < VR && oy < H && ox < W) {
+        if (oy < H && ox < W) {
             const size_t p = (size_t)oy * W + ox;
             if (chunked) {
                 float *pc = P.part + (size_t)chunk * kPartPlanes * HW;
@@ -432,7 +423,6 @@ __global__ void sweep_finalize_kernel(SweepK P, float *depth)
     depth[p] = (float)(1.0 / m);
 }
 
-static int tile_height(int r) { return r <= kMaxR ? ((kHRows - 2 * r) / kVSeg) * kVSeg : 0; }
 
 static int build_params(const om_sweep_desc *d, SweepK &P)
 {
@@ -487,9 +477,9 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
         if (o.geo[13] != 0.0 || o.geo[14] != 0.0 || o.geo[15] != 0.0)
             P.pinhole = 0;
     }
-    P.TH = d->tile_h > 0 ? d->tile_h : tile_height(d->r);
-    if (P.TH < kVSeg || P.TH % kVSeg || P.TH + 2 * d->r > kHRows || P.TH / kVSeg > 8)
-        return fail(OM_EINVAL, "tile height %d invalid for r=%d", P.TH, d->r);
+    if (d->tile_h > 0 && d->tile_h != kTH)
+        return fail(OM_EINVAL, "tile height is fixed at %d rows (got %d)", kTH, d->tile_h);
+    P.TH = kTH;
     P.tiles_x = (int)cdiv(d->W, kTW);
     P.tiles_y = (int)cdiv(d->H, P.TH);
     return OM_OK;
@@ -545,16 +535,14 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
     Pl.part = (float *)d->workspace;
     const int r = Pl.r;
     const size_t halo = (size_t)(Pl.TH + 2 * r) * (kTW + 2 * r);
-    const size_t lds = sizeof(float) * (2 * halo + 4 * (size_t)(Pl.TH + 2 * r) * kTW);
+    const size_t lds = halo * (sizeof(double) + sizeof(int) + 2 * sizeof(float)) +
+                       sizeof(float) * 4 * (size_t)(Pl.TH + 2 * r) * kTW;
     const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
     hipStream_t st = as_stream(stream);
     ProfScope ps("sweep", st);
-    switch (Pl.k) {
-    case 1: hipLaunchKernelGGL(sweep_kernel<1>, grid, dim3(kThreads), lds, st, Pl); break;
-    case 2: hipLaunchKernelGGL(sweep_kernel<2>, grid, dim3(kThreads), lds, st, Pl); break;
-    case 3: hipLaunchKernelGGL(sweep_kernel<3>, grid, dim3(kThreads), lds, st, Pl); break;
-    default: hipLaunchKernelGGL(sweep_kernel<4>, grid, dim3(kThreads), lds, st, Pl); break;
-    }
+    using KFn = void (*)(const SweepK);
+    static const KFn table[4] = {sweep_kernel<1>, sweep_kernel<2>, sweep_kernel<3>, sweep_kernel<4>};
+    hipLaunchKernelGGL(table[Pl.k - 1], grid, dim3(kThreads), lds, st, Pl);
     if ((rc = launch_check("sweep_kernel")))
         return rc;
     if (Pl.chunks > 1) {
