@@ -98,17 +98,30 @@ struct PMK {
     int *row_cnt, *row_off;    // (H,)
     uint32_t *st_in, *st_out;  // stream state before / after the pass
     unsigned long long *changed;
-    const float *img[16];
-    const float *dep[16];
-    int ih[16], iw[16];
-    double par[16][18];
+    const struct PMCam *cam;   // (S,) per-source tables in device memory
+};
+
+// One source view as the cost kernels read it (device memory, uniform loads): the reference's
+// par row (R 9, C 3, f, k1, k2, k3, cx, cy), image, optional depth map, size.
+struct PMCam {
+    double par[18];
+    const float *img;
+    const float *dep;
+    int iw, ih;
+    int pad[2];
+};
+
+// Host-side staging of the source tables, passed by value to pm_setup_kernel.
+struct PMCamArgs {
+    PMCam cam[16];
+    int S;
 };
 
 // world point -> source pixel (_dense.pyx:429-445)
 __device__ __forceinline__ bool pm_proj(const PMK &P, int s, double X, double Y, double Z, double &u, double &v,
                                         double &z)
 {
-    const double *p = P.par[s];
+    const double *p = P.cam[s].par;
     const double dx = X - p[9], dy = Y - p[10], dz = Z - p[11];
     const double xc = p[0] * dx + p[1] * dy + p[2] * dz;
     const double yc = p[3] * dx + p[4] * dy + p[5] * dz;
@@ -141,6 +154,19 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, int h, double u
            ((double)p[w] * (1.0 - fx) + (double)p[w + 1] * fx) * fy;
 }
 
+#ifndef OM_PM_GROUP
+#define OM_PM_GROUP 4
+#endif
+constexpr int kGroup = OM_PM_GROUP;  // sources accumulated together in the cost's sample loop
+constexpr int kPMThreads = 128;
+
+// per-thread LDS slots for the S source scores of one cost evaluation
+__device__ __forceinline__ double *pm_ncc_scratch()
+{
+    __shared__ double ncc_lds[kPMThreads * 16];
+    return ncc_lds + threadIdx.x * 16;
+}
+
 // slanted-window cost (_dense.pyx:461-556); photo is written only on the full path
 template <int SM>
 __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, double n1, double n2, double &photo)
@@ -152,71 +178,85 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
     if (ndr >= -1e-6 || d <= 0)
         return 10.0;
     const double num = d * ndr;
-    double sI[SM], sJ[SM], sII[SM], sJJ[SM], sIJ[SM];
-    int cnt[SM];
-#pragma unroll
-    for (int s = 0; s < SM; ++s) {
-        sI[s] = sJ[s] = sII[s] = sJJ[s] = sIJ[s] = 0.0;
-        cnt[s] = 0;
-    }
-    int nsamp = 0;
     const int r = P.r, st = P.step, S = P.S;
-    for (int a = -r; a <= r; a += st) {
-        const int yy = y + a;
-        if (yy < 0 || yy >= H)
-            continue;
-        for (int b = -r; b <= r; b += st) {
-            const int xx = x + b;
-            if (xx < 0 || xx >= W)
-                continue;
-            const size_t q = (size_t)yy * W + xx;
-            const double I = P.ref[q];
-            if (!isfinite(I))
-                continue;
-            const double qx = P.ray[q], qy = P.ray[HW + q], qz = P.ray[2 * HW + q];
-            const double den = n0 * qx + n1 * qy + n2 * qz;
-            if (den >= -1e-9)
-                continue;
-            const double t = num / den;
-            const double X = P.cr0 + t * qx, Y = P.cr1 + t * qy, Z = P.cr2 + t * qz;
-            nsamp += 1;
+    double *ncc_l = pm_ncc_scratch();  // this thread's S NCC values (LDS)
+    int nsamp = 0;
+    // Sources in groups of kGroup, the group loop kept rolled: per-sample geometry is recomputed
+    // once per group, only kGroup sets of accumulators and one group's camera parameters are live.
+    // Each source still sums its samples in window order, so every sum is the reference's.
+#pragma unroll 1
+    for (int g0 = 0; g0 < S; g0 += kGroup) {
+        double sI[kGroup], sJ[kGroup], sII[kGroup], sJJ[kGroup], sIJ[kGroup];
+        int cnt[kGroup];
 #pragma unroll
-            for (int s = 0; s < SM; ++s) {
-                if (s < S) {
-                    double u, v, z;
-                    if (pm_proj(P, s, X, Y, Z, u, v, z)) {
-                        const double J = pm_bil(P.img[s], P.iw[s], P.ih[s], u, v);
-                        if (isfinite(J)) {
-                            sI[s] += I;
-                            sJ[s] += J;
-                            sII[s] += I * I;
-                            sJJ[s] += J * J;
-                            sIJ[s] += I * J;
-                            cnt[s] += 1;
+        for (int u = 0; u < kGroup; ++u) {
+            sI[u] = sJ[u] = sII[u] = sJJ[u] = sIJ[u] = 0.0;
+            cnt[u] = 0;
+        }
+        nsamp = 0;
+        for (int a = -r; a <= r; a += st) {
+            const int yy = y + a;
+            if (yy < 0 || yy >= H)
+                continue;
+            for (int b = -r; b <= r; b += st) {
+                const int xx = x + b;
+                if (xx < 0 || xx >= W)
+                    continue;
+                const size_t q = (size_t)yy * W + xx;
+                const double I = P.ref[q];
+                if (!isfinite(I))
+                    continue;
+                const double qx = P.ray[q], qy = P.ray[HW + q], qz = P.ray[2 * HW + q];
+                const double den = n0 * qx + n1 * qy + n2 * qz;
+                if (den >= -1e-9)
+                    continue;
+                const double t = num / den;
+                const double X = P.cr0 + t * qx, Y = P.cr1 + t * qy, Z = P.cr2 + t * qz;
+                nsamp += 1;
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    const int s = g0 + u;
+                    if (s < S) {
+                        double pu, pv, pz;
+                        if (pm_proj(P, s, X, Y, Z, pu, pv, pz)) {
+                            const PMCam &cs = P.cam[s];
+                            const double J = pm_bil(cs.img, cs.iw, cs.ih, pu, pv);
+                            if (isfinite(J)) {
+                                sI[u] += I;
+                                sJ[u] += J;
+                                sII[u] += I * I;
+                                sJJ[u] += J * J;
+                                sIJ[u] += I * J;
+                                cnt[u] += 1;
+                            }
                         }
                     }
                 }
             }
         }
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+            const int s = g0 + u;
+            if (s < S) {
+                double v;
+                if (nsamp < 4 || cnt[u] < 0.7 * nsamp) {
+                    v = -1.0;
+                } else {
+                    const double m = (double)cnt[u];
+                    const double mi = sI[u] / m, mj = sJ[u] / m;
+                    const double vi = sII[u] / m - mi * mi;
+                    const double vj = sJJ[u] / m - mj * mj;
+                    const double cv = sIJ[u] / m - mi * mj;
+                    v = (vi < 1e-8 || vj < 1e-8) ? 0.0 : cv / sqrt(vi * vj);
+                }
+                ncc_l[s] = v;
+            }
+        }
     }
     double ncc[SM];
 #pragma unroll
-    for (int s = 0; s < SM; ++s) {
-        if (s < S) {
-            if (nsamp < 4 || cnt[s] < 0.7 * nsamp) {
-                ncc[s] = -1.0;
-            } else {
-                const double m = (double)cnt[s];
-                const double mi = sI[s] / m, mj = sJ[s] / m;
-                const double vi = sII[s] / m - mi * mi;
-                const double vj = sJJ[s] / m - mj * mj;
-                const double cv = sIJ[s] / m - mi * mj;
-                ncc[s] = (vi < 1e-8 || vj < 1e-8) ? 0.0 : cv / sqrt(vi * vj);
-            }
-        } else {
-            ncc[s] = 0.0;
-        }
-    }
+    for (int s = 0; s < SM; ++s)
+        ncc[s] = s < S ? ncc_l[s] : 0.0;
     // descending insertion sort of the S values, the reference's exact procedure (:520-526)
 #pragma unroll
     for (int a = 1; a < SM; ++a) {
@@ -251,13 +291,14 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
         int ng = 0;
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
-            if (s < S && P.dep[s]) {
+            const PMCam &cs = P.cam[s < S ? s : 0];
+            if (s < S && cs.dep) {
                 ng += 1;
                 double e = 1.0, u, v, z;
                 if (pm_proj(P, s, X, Y, Z, u, v, z)) {
                     const int ui = (int)(u + 0.5), vi = (int)(v + 0.5);
-                    if (ui >= 0 && vi >= 0 && ui < P.iw[s] && vi < P.ih[s]) {
-                        const double dj = P.dep[s][(size_t)vi * P.iw[s] + ui];
+                    if (ui >= 0 && vi >= 0 && ui < cs.iw && vi < cs.ih) {
+                        const double dj = cs.dep[(size_t)vi * cs.iw + ui];
                         if (isfinite(dj) && dj > 0) {
                             e = fabs(z - dj) / dj / P.geo_tol;
                             if (e > 1)
@@ -291,7 +332,7 @@ __device__ __forceinline__ void count_changes(const PMK &P, int n)
 
 // A: current plane + propagation from the 8 neighbours (:614-645)
 template <int SM>
-__global__ void __launch_bounds__(128) pm_propagate_kernel(const PMK P)
+__global__ void __launch_bounds__(kPMThreads) pm_propagate_kernel(const PMK P)
 {
     static const int oy[8] = {-1, 1, 0, 0, -5, 5, 0, 0};
     static const int ox[8] = {0, 0, -1, 1, 0, 0, -5, 5};
@@ -412,7 +453,7 @@ __global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P)
 
 // C: the 2 random refinements with the pixel's own slice of the stream (:646-664); score write
 template <int SM>
-__global__ void __launch_bounds__(128) pm_random_kernel(const PMK P)
+__global__ void __launch_bounds__(kPMThreads) pm_random_kernel(const PMK P)
 {
     int y, x;
     if (!colour_pixel(P, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, y, x))
@@ -453,10 +494,13 @@ __global__ void __launch_bounds__(128) pm_random_kernel(const PMK P)
     count_changes(P, changed);
 }
 
-__global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed)
+__global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed, const PMCamArgs A,
+                               PMCam *cam)
 {
     st[0] = seed | 1u;
     *changed = 0ull;
+    for (int s = 0; s < A.S; ++s)
+        cam[s] = A.cam[s];
 }
 
 // workspace layout
@@ -465,6 +509,7 @@ struct PMWs {
     int *rank, *row_cnt, *row_off;
     uint32_t *st;  // 2 states
     unsigned long long *changed;
+    PMCam *cam;    // 16 source tables
 };
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -479,7 +524,7 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w)
         return p;
     };
     char *b = take(HW * 8), *s = take(HW * 8), *rk = take(HW * 4), *rc = take((size_t)H * 4),
-         *ro = take((size_t)H * 4), *st = take(16), *ch = take(8);
+         *ro = take((size_t)H * 4), *st = take(16), *ch = take(8), *cm = take(sizeof(PMCam) * 16);
     if (w) {
         w->best = (double *)b;
         w->sc = (double *)s;
@@ -488,16 +533,18 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w)
         w->row_off = (int *)ro;
         w->st = (uint32_t *)st;
         w->changed = (unsigned long long *)ch;
+        w->cam = (PMCam *)cm;
     }
     return o;
 }
 
 template <int SM>
-static int pm_run(PMK &P, int iters, hipStream_t stream, uint32_t seed, const PMWs &w)
+static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uint32_t seed, const PMWs &w)
 {
     const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
-    const unsigned gA = (unsigned)cdiv(ncol, 128);
-    hipLaunchKernelGGL(pm_init_kernel, dim3(1), dim3(1), 0, stream, w.st, seed, w.changed);
+    const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
+    P.cam = w.cam;
+    hipLaunchKernelGGL(pm_init_kernel, dim3(1), dim3(1), 0, stream, w.st, seed, w.changed, A, w.cam);
     int pass = 0;
     for (int it = 0; it < iters; ++it) {
         P.dsc = 0.05 / (it + 1);
@@ -508,7 +555,7 @@ static int pm_run(PMK &P, int iters, hipStream_t stream, uint32_t seed, const PM
             P.st_out = w.st + ((pass + 1) & 1);
             {
                 ProfScope ps("pm_propagate", stream);
-                hipLaunchKernelGGL(pm_propagate_kernel<SM>, dim3(gA), dim3(128), 0, stream, P);
+                hipLaunchKernelGGL(pm_propagate_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P);
             }
             {
                 ProfScope ps("pm_rank_rows", stream);
@@ -520,7 +567,7 @@ static int pm_run(PMK &P, int iters, hipStream_t stream, uint32_t seed, const PM
             }
             {
                 ProfScope ps("pm_random", stream);
-                hipLaunchKernelGGL(pm_random_kernel<SM>, dim3(gA), dim3(128), 0, stream, P);
+                hipLaunchKernelGGL(pm_random_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P);
             }
         }
     }
@@ -592,28 +639,31 @@ extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float 
     P.row_off = w.row_off;
     P.changed = w.changed;
     P.pinhole = 1;
+    PMCamArgs A;
+    memset(&A, 0, sizeof A);
+    A.S = S;
     for (int s = 0; s < S; ++s) {
         if (ih[s] < 1 || iw[s] < 1)
             return fail(OM_ESHAPE, "source %d has an empty image", s);
-        P.img[s] = imgs[s];
-        P.dep[s] = deps[s];
-        P.ih[s] = ih[s];
-        P.iw[s] = iw[s];
+        A.cam[s].img = imgs[s];
+        A.cam[s].dep = deps[s];
+        A.cam[s].ih = ih[s];
+        A.cam[s].iw = iw[s];
         for (int t = 0; t < 18; ++t)
-            P.par[s][t] = par[18 * s + t];
+            A.cam[s].par[t] = par[18 * s + t];
         if (par[18 * s + 13] != 0.0 || par[18 * s + 14] != 0.0 || par[18 * s + 15] != 0.0)
             P.pinhole = 0;
     }
     hipStream_t st = as_stream(stream);
     int rc;
     if (S <= 4)
-        rc = pm_run<4>(P, iters, st, seed, w);
+        rc = pm_run<4>(P, A, iters, st, seed, w);
     else if (S <= 8)
-        rc = pm_run<8>(P, iters, st, seed, w);
+        rc = pm_run<8>(P, A, iters, st, seed, w);
     else if (S <= 12)
-        rc = pm_run<12>(P, iters, st, seed, w);
+        rc = pm_run<12>(P, A, iters, st, seed, w);
     else
-        rc = pm_run<16>(P, iters, st, seed, w);
+        rc = pm_run<16>(P, A, iters, st, seed, w);
     if (rc)
         return rc;
     if (changed) {
