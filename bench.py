@@ -68,9 +68,29 @@ def prof_read(lib, name):
 
 
 # ------------------------------------------------------------------------------- C2 PatchMatch
-def setup_c2(torch, scale=1.0):
+def _scene_c2(torch, cache):
+    """The C2 scene, rendered on the device or loaded from `cache` (written on first use), so that
+    profiled runs launch only the depth-map kernels."""
     from py_ortho_amd import scene
-    sc = scene.config_c2(device="cuda", scale=scale)
+    if cache and os.path.exists(cache):
+        z = np.load(cache, allow_pickle=False)
+        sc = scene.config_c2_cameras()
+        views = [sc.ref] + sc.srcs
+        for k, v in enumerate(views):
+            v.img = torch.from_numpy(z[f"img{k}"]).cuda()
+            v.depth = torch.from_numpy(z[f"dep{k}"]).cuda()
+        return sc
+    sc = scene.config_c2(device="cuda")
+    if cache:
+        views = [sc.ref] + sc.srcs
+        np.savez(cache, **{f"img{k}": v.img.cpu().numpy() for k, v in enumerate(views)},
+                 **{f"dep{k}": v.depth.cpu().numpy() for k, v in enumerate(views)})
+    return sc
+
+
+def setup_c2(torch, cache=None):
+    from py_ortho_amd import scene
+    sc = _scene_c2(torch, cache)
     H, W = sc.ref.img.shape
     ray = scene.rays_world(sc.ref.cam, "cuda")
     depth0, normal0 = scene.pm_init(3, H, W, ray, 80.0, 105.0)
@@ -85,7 +105,7 @@ def setup_c2(torch, scale=1.0):
 def run_c2(args, torch):
     from py_ortho_amd import _lib, dense
     lib = _lib.lib()
-    c = setup_c2(torch)
+    c = setup_c2(torch, args.scene_cache)
     H, W, S = c["H"], c["W"], len(c["srcs"])
     depth = torch.empty_like(c["depth0"])
     normal = torch.empty_like(c["normal0"])
@@ -259,6 +279,10 @@ def main(argv=None):
     ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = all cores)")
+    ap.add_argument("--scene-cache", default=None, help="npz file holding the rendered C2 scene (created if absent)")
+    ap.add_argument("--calib-mb", type=int, default=0,
+                    help="first run one plain elementwise kernel over this many MiB (read N, write N) as the "
+                         "reference point for hardware byte counters")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (views shard across "
@@ -268,6 +292,13 @@ def main(argv=None):
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+    if args.calib_mb > 0:
+        n = args.calib_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     res = run_c2(args, torch) if args.workload == "c2" else run_c1(args, torch)
     print(json.dumps(res), flush=True)
     return 0

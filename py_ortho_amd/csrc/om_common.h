@@ -38,6 +38,20 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with its own L2.
+// xcd_block maps the hardware block index to a logical one so that every XCD walks a contiguous
+// range of logical blocks (a band of the image), which keeps its working set in its own L2.
+constexpr int kXCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int nblocks)
+{
+#ifdef OM_NO_XCD_REMAP
+    return b;
+#endif
+    const int xcd = b % kXCD, k = b / kXCD;
+    const int per = nblocks / kXCD, extra = nblocks % kXCD;
+    return xcd < extra ? xcd * (per + 1) + k : extra * (per + 1) + (xcd - extra) * per + k;
+}
+
 // Optional per-launch timing (om_profile_enable): HIP events recorded on the launch stream
 // around each instrumented kernel, read back by om_profile_read.  Off by default; when off a
 // scope costs one relaxed atomic load.

@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(kPMThreads) pm_propagate_kernel(const PMK P)
     static const int oy[8] = {-1, 1, 0, 0, -5, 5, 0, 0};
     static const int ox[8] = {0, 0, -1, 1, 0, 0, -5, 5};
     int y, x;
-    if (!colour_pixel(P, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, y, x))
+    if (!colour_pixel(P, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, y, x))
         return;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])
@@ -456,7 +456,7 @@ template <int SM>
 __global__ void __launch_bounds__(kPMThreads) pm_random_kernel(const PMK P)
 {
     int y, x;
-    if (!colour_pixel(P, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, y, x))
+    if (!colour_pixel(P, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, y, x))
         return;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])

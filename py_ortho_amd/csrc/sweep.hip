@@ -120,8 +120,9 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
 
     const int tid = threadIdx.x;
     const int ntiles = P.tiles_x * P.tiles_y;
-    const int tile = blockIdx.x % ntiles;
-    const int chunk = blockIdx.x / ntiles;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
+    const int tile = lb / P.chunks;
+    const int chunk = lb % P.chunks;
     const int tx0 = (tile % P.tiles_x) * kTW;
     const int ty0 = (tile / P.tiles_x) * TH;
     const int ia = P.i0 + chunk * P.hc;

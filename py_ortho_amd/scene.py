@@ -60,12 +60,14 @@ class Terrain:
     octaves: Tuple[Tuple[float, float], ...] = ((8.0, 0.5), (2.0, 0.25), (0.5, 0.15), (0.13, 0.1))
 
     def height_at(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
-        dev = X.device
-        z = torch.zeros_like(X)
-        for k in range(len(self.height)):
-            d2 = (X - float(self.cx[k])) ** 2 + (Y - float(self.cy[k])) ** 2
-            z = z + float(self.height[k]) * torch.exp(-d2 / (2.0 * float(self.sigma[k]) ** 2))
-        del dev
+        """Sum of the bumps at (X, Y), clipped to [0, hmax] (all bumps in one broadcast)."""
+        dev, dt = X.device, X.dtype
+        cx = torch.as_tensor(self.cx, dtype=dt, device=dev)
+        cy = torch.as_tensor(self.cy, dtype=dt, device=dev)
+        h = torch.as_tensor(self.height, dtype=dt, device=dev)
+        inv2s2 = torch.as_tensor(1.0 / (2.0 * np.asarray(self.sigma) ** 2), dtype=dt, device=dev)
+        d2 = (X[..., None] - cx) ** 2 + (Y[..., None] - cy) ** 2
+        z = (h * torch.exp(-d2 * inv2s2)).sum(-1)
         return torch.clamp(z, 0.0, self.hmax)
 
 
@@ -115,7 +117,7 @@ def camera_rays(cam: Camera, device, dtype=torch.float64) -> torch.Tensor:
 
 
 def render(terr: Terrain, cam: Camera, noise_seed: int, noise: float = 0.01, device="cpu", march: int = 96,
-           bisect: int = 40, rows_per_chunk: int = 256):
+           bisect: int = 40, rows_per_chunk: int = 1024):
     """-> (image (h, w) float32 in [0,1] + noise, depth (h, w) float32 camera-axis)."""
     img = torch.empty((cam.h, cam.w), dtype=torch.float32, device=device)
     dep = torch.empty((cam.h, cam.w), dtype=torch.float32, device=device)
@@ -199,11 +201,26 @@ def config_c1(device="cpu", scale: float = 1.0) -> Scene:
     return make_scene(0, W, H, 600.0 * scale, 80.0, [(b, 0.0), (-b, 0.0), (0.0, b), (0.0, -b)], device=device)
 
 
+C2_SOURCES = [(x, y) for y in (-10.0, 10.0) for x in (-30.0, -10.0, 10.0, 30.0)]
+
+
 def config_c2(device="cpu", scale: float = 1.0) -> Scene:
     """Single-view PatchMatch: 2000x1500, f=1800, 100 m, 8 sources on a 2x4 grid, 20 m baseline."""
     W, H = int(round(2000 * scale)), int(round(1500 * scale))
-    src = [(x, y) for y in (-10.0, 10.0) for x in (-30.0, -10.0, 10.0, 30.0)]
-    return make_scene(3, W, H, 1800.0 * scale, 100.0, src, device=device)
+    return make_scene(3, W, H, 1800.0 * scale, 100.0, C2_SOURCES, device=device)
+
+
+def config_c2_cameras(scale: float = 1.0) -> Scene:
+    """The cameras and terrain of config_c2 without rendering (images filled in by the caller)."""
+    W, H = int(round(2000 * scale)), int(round(1500 * scale))
+    f, alt = 1800.0 * scale, 100.0
+    half_w, half_h = (W / 2) / f * alt, (H / 2) / f * alt
+    xs = [p[0] for p in C2_SOURCES] + [0.0]
+    ys = [p[1] for p in C2_SOURCES] + [0.0]
+    terr = make_terrain(3, (min(xs) - half_w, max(xs) + half_w, min(ys) - half_h, max(ys) + half_h))
+    cams = [pinhole((0.0, 0.0, alt), f, W, H)] + [pinhole((x, y, alt), f, W, H) for x, y in C2_SOURCES]
+    views = [View(c, None, None) for c in cams]
+    return Scene(terr, views[0], views[1:])
 
 
 def pm_init(seed: int, H: int, W: int, ray: torch.Tensor, dmin: float, dmax: float, max_tilt_deg: float = 30.0):
