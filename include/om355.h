@@ -130,6 +130,58 @@ int om_half(const float *g, int32_t H, int32_t W, float *out, void *stream);
 int om_rays_world(int32_t H, int32_t W, const double *R, double cx, double cy, double f, double k1, double k2,
                   double k3, float *out, void *stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Batches: many views' depth maps planned once and launched together (no counterpart call in
+ * the reference, which runs one view per host thread, densify.py:596).  A batch owns device
+ * descriptor tables and workspaces, allocated by the om_batch_set_* calls; om_batch_run only
+ * enqueues kernels, so it can be captured in a HIP graph (om_batch_capture) and replayed.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct om_batch om_batch;
+
+/* one view of a batched PatchMatch: the arguments of om_pm_refine for that view */
+typedef struct om_pm_view {
+    const float *ref;              /* (H, W), NaN = masked */
+    int32_t H, W;
+    const float *ray;              /* (3, H, W) */
+    double Cr[3];
+    int32_t S;                     /* 1..16 */
+    const float *imgs[16];         /* device images */
+    const float *deps[16];         /* device depth maps or NULL */
+    int32_t ih[16], iw[16];
+    double par[16][18];            /* R (9), C (3), f, k1, k2, k3, cx, cy per source */
+    float *depth, *normal, *score; /* in/out */
+    const uint8_t *active;         /* (H, W) */
+    double geo_tol, dmin, dmax;
+    uint32_t seed;
+    /* optional start maps (device, NULL = none): every run first copies depth_from (H, W) into
+     * depth and score_from (H, W) into score; with init_normals the normals come from the start depth */
+    const float *depth_from, *score_from;
+} om_pm_view;
+
+om_batch *om_batch_create(void);
+int om_batch_destroy(om_batch *b);
+/* sweep of nviews views sharing r and min(top_k, n_src) (each desc's workspace/chunks ignored);
+ * every run resets each view's winner state and sweeps its hypotheses [0, nh) (i0 must be 0);
+ * depth_out[v] (array may be NULL, entries may be NULL) receives view v's parabola depth;
+ * chunks: hypothesis chunks per view (0 = auto); tile_h: 8 or 16 (0 = 16); grid_blocks: persistent
+ * workgroups (0 = all resident). */
+int om_batch_set_sweep(om_batch *b, const om_sweep_desc *views, int32_t nviews, float *const *depth_out,
+                       int32_t chunks, int32_t tile_h, int32_t grid_blocks);
+/* PatchMatch over nviews views after the sweep (same r, step, top_k, geo_w, iters for all);
+ * init_normals: first set each view's normals from its depth (densify._init_normals, densify.py:481-496). */
+int om_batch_set_pm(om_batch *b, const om_pm_view *views, int32_t nviews, int32_t r, int32_t step, int32_t top_k,
+                    double geo_w, int32_t iters, int32_t init_normals);
+/* enqueue the batch's kernels on stream: the sweep (+merge, parabola), then the PatchMatch passes */
+int om_batch_run(om_batch *b, void *stream);
+/* record om_batch_run into a HIP graph (stream capture on `stream`) and instantiate it */
+int om_batch_capture(om_batch *b, void *stream);
+int om_batch_replay(om_batch *b, void *stream);
+/* accepted PatchMatch changes per view (synchronises the device) */
+int om_batch_pm_changed(om_batch *b, int64_t *changed);
+
+/* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
+int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);
+
 /* height-sweep backend operations (orthomosaic/backend.py:67-99) */
 /* _mvs.sample_view (_mvs.pyx:13-54) over `planes` height maps Z (planes, H, W): img (h, w, nch)
  * float32; out (planes, H, W, nch) must be zeroed by the caller (untouched where invalid). */

@@ -42,6 +42,16 @@ class OmSweepDesc(ctypes.Structure):
     ]
 
 
+class OmPmView(ctypes.Structure):
+    _fields_ = [
+        ("ref", c_void_p), ("H", c_int32), ("W", c_int32), ("ray", c_void_p), ("Cr", c_double * 3),
+        ("S", c_int32), ("imgs", c_void_p * 16), ("deps", c_void_p * 16), ("ih", c_int32 * 16), ("iw", c_int32 * 16),
+        ("par", (c_double * 18) * 16), ("depth", c_void_p), ("normal", c_void_p), ("score", c_void_p),
+        ("active", c_void_p), ("geo_tol", c_double), ("dmin", c_double), ("dmax", c_double),
+        ("seed", ctypes.c_uint32), ("depth_from", c_void_p), ("score_from", c_void_p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/om355.h
 SIGNATURES = {
     "om_version": (c_int32, []),
@@ -81,6 +91,17 @@ SIGNATURES = {
                                   c_void_p]),
     "om_topk_mean": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p, c_void_p]),
     "om_argminmax0": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
+    "om_batch_create": (c_void_p, []),
+    "om_batch_destroy": (c_int32, [c_void_p]),
+    "om_batch_set_sweep": (c_int32, [c_void_p, ctypes.POINTER(OmSweepDesc), c_int32, ctypes.POINTER(c_void_p),
+                                     c_int32, c_int32, c_int32]),
+    "om_batch_set_pm": (c_int32, [c_void_p, ctypes.POINTER(OmPmView), c_int32, c_int32, c_int32, c_int32, c_double,
+                                  c_int32, c_int32]),
+    "om_batch_run": (c_int32, [c_void_p, c_void_p]),
+    "om_batch_capture": (c_int32, [c_void_p, c_void_p]),
+    "om_batch_replay": (c_int32, [c_void_p, c_void_p]),
+    "om_batch_pm_changed": (c_int32, [c_void_p, ctypes.POINTER(c_int64)]),
+    "om_init_normals": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -91,14 +91,15 @@ struct PMK {
     const float *ref, *ray;
     const uint8_t *active;
     float *depth, *normal, *score;
-    int H, W, S, r, step, kk, pinhole, colour;
-    double cr0, cr1, cr2, geo_w, geo_tol, dmin, dmax, dsc, nsc;
+    int H, W, S, r, step, kk, pinhole, seed;
+    double cr0, cr1, cr2, geo_w, geo_tol, dmin, dmax;
     double *best, *sc;         // (H, W) per-pixel cost and photometric score of the pass
     int *rank;                 // (H, W) draw rank of the pixel within its row, -1 = no draws
     int *row_cnt, *row_off;    // (H,)
-    uint32_t *st_in, *st_out;  // stream state before / after the pass
+    uint32_t *st;              // 2 stream states: before / after the current pass (by parity)
     unsigned long long *changed;
     const struct PMCam *cam;   // (S,) per-source tables in device memory
+    const float *depth_from, *score_from;  // batched runs: start maps copied in first (or NULL)
 };
 
 // One source view as the cost kernels read it (device memory, uniform loads): the reference's
@@ -315,12 +316,18 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
     return (1.0 - ph) + P.geo_w * geo;
 }
 
-__device__ __forceinline__ bool colour_pixel(const PMK &P, int64_t t, int &y, int &x)
+// what changes from one colour pass to the next
+struct PassK {
+    int colour, parity;
+    double dsc, nsc;  // 0.05 / (it+1), 0.3 / (it+1) (_dense.pyx:609-610)
+};
+
+__device__ __forceinline__ bool colour_pixel(const PMK &P, const PassK &Q, int64_t t, int &y, int &x)
 {
     const int W2 = (P.W + 1) / 2;
     y = (int)(t / W2);
     const int j = (int)(t - (int64_t)y * W2);
-    x = 2 * j + ((P.colour + y) & 1);
+    x = 2 * j + ((Q.colour + y) & 1);
     return y < P.H && x < P.W;
 }
 
@@ -330,14 +337,14 @@ __device__ __forceinline__ void count_changes(const PMK &P, int n)
         atomicAdd(P.changed, (unsigned long long)n);
 }
 
-// A: current plane + propagation from the 8 neighbours (:614-645)
+// A: current plane + propagation from the 8 neighbours (:614-645), pixel t of the colour
 template <int SM>
-__global__ void __launch_bounds__(kPMThreads) pm_propagate_kernel(const PMK P)
+__device__ __forceinline__ void pm_propagate_px(const PMK &P, const PassK &Q, int64_t t)
 {
     static const int oy[8] = {-1, 1, 0, 0, -5, 5, 0, 0};
     static const int ox[8] = {0, 0, -1, 1, 0, 0, -5, 5};
     int y, x;
-    if (!colour_pixel(P, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, y, x))
+    if (!colour_pixel(P, Q, t, y, x))
         return;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])
@@ -386,14 +393,14 @@ __global__ void __launch_bounds__(kPMThreads) pm_propagate_kernel(const PMK P)
     count_changes(P, changed);
 }
 
-// B1: per row, the rank of every drawing pixel (active, this colour, finite depth) in x order
-__global__ void __launch_bounds__(256) pm_rank_rows_kernel(const PMK P)
+// B1: for row y, the rank of every drawing pixel (active, this colour, finite depth) in x order
+// (a whole workgroup of 256 threads per row)
+__device__ __forceinline__ void pm_rank_row(const PMK &P, const PassK &Q, int y)
 {
     __shared__ int wsum[4];
     __shared__ int carry_s;
-    const int y = blockIdx.x;
     const int W2 = (P.W + 1) / 2;
-    const int x0 = (P.colour + y) & 1;
+    const int x0 = (Q.colour + y) & 1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x == 0)
         carry_s = 0;
@@ -423,7 +430,8 @@ __global__ void __launch_bounds__(256) pm_rank_rows_kernel(const PMK P)
 }
 
 // B2: exclusive scan of the row counts; stream state after the pass = jump(state, 8 * total)
-__global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P)
+// (one workgroup of 256 threads per view)
+__device__ __forceinline__ void pm_scan_view(const PMK &P, const PassK &Q)
 {
     __shared__ long long part[256];
     const int H = P.H;
@@ -441,7 +449,7 @@ __global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P)
             part[t] = acc;
             acc += v;
         }
-        *P.st_out = xs_jump(*P.st_in, 8ull * (unsigned long long)acc);
+        P.st[Q.parity ^ 1] = xs_jump(P.st[Q.parity], 8ull * (unsigned long long)acc);
     }
     __syncthreads();
     long long off = part[threadIdx.x];
@@ -453,10 +461,10 @@ __global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P)
 
 // C: the 2 random refinements with the pixel's own slice of the stream (:646-664); score write
 template <int SM>
-__global__ void __launch_bounds__(kPMThreads) pm_random_kernel(const PMK P)
+__device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64_t t)
 {
     int y, x;
-    if (!colour_pixel(P, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, y, x))
+    if (!colour_pixel(P, Q, t, y, x))
         return;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])
@@ -465,13 +473,13 @@ __global__ void __launch_bounds__(kPMThreads) pm_random_kernel(const PMK P)
     const int rk = P.rank[p];
     int changed = 0;
     if (rk >= 0) {
-        uint32_t st = xs_jump(*P.st_in, 8ull * (unsigned long long)(P.row_off[y] + rk));
+        uint32_t st = xs_jump(P.st[Q.parity], 8ull * (unsigned long long)(P.row_off[y] + rk));
         double photo = -1.0;
         for (int tr = 0; tr < 2; ++tr) {
-            const double dp = (double)P.depth[p] * (1 + P.dsc * (2 * xs_rnd(st) - 1));
-            double b0 = (double)P.normal[p] + P.nsc * (2 * xs_rnd(st) - 1);
-            double b1 = (double)P.normal[HW + p] + P.nsc * (2 * xs_rnd(st) - 1);
-            double b2 = (double)P.normal[2 * HW + p] + P.nsc * (2 * xs_rnd(st) - 1);
+            const double dp = (double)P.depth[p] * (1 + Q.dsc * (2 * xs_rnd(st) - 1));
+            double b0 = (double)P.normal[p] + Q.nsc * (2 * xs_rnd(st) - 1);
+            double b1 = (double)P.normal[HW + p] + Q.nsc * (2 * xs_rnd(st) - 1);
+            double b2 = (double)P.normal[2 * HW + p] + Q.nsc * (2 * xs_rnd(st) - 1);
             const double nn = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
             if (nn < 1e-9 || dp < P.dmin || dp > P.dmax)
                 continue;
@@ -494,6 +502,23 @@ __global__ void __launch_bounds__(kPMThreads) pm_random_kernel(const PMK P)
     count_changes(P, changed);
 }
 
+// ---------------------------------------------------------------- single-view kernels
+template <int SM>
+__global__ void __launch_bounds__(kPMThreads) pm_propagate_kernel(const PMK P, const PassK Q)
+{
+    pm_propagate_px<SM>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) pm_rank_rows_kernel(const PMK P, const PassK Q) { pm_rank_row(P, Q, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P, const PassK Q) { pm_scan_view(P, Q); }
+
+template <int SM>
+__global__ void __launch_bounds__(kPMThreads) pm_random_kernel(const PMK P, const PassK Q)
+{
+    pm_random_px<SM>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+}
+
 __global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed, const PMCamArgs A,
                                PMCam *cam)
 {
@@ -501,6 +526,60 @@ __global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *
     *changed = 0ull;
     for (int s = 0; s < A.S; ++s)
         cam[s] = A.cam[s];
+}
+
+// ---------------------------------------------------------------- batched kernels (many views)
+// blocks of all views in one grid: view v owns logical blocks [blk0[v], blk0[v+1])
+__device__ __forceinline__ int view_of(const int *blk0, int nviews, int b)
+{
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int SM>
+__global__ void __launch_bounds__(kPMThreads) pm_propagate_batch(const PMK *views, const int *blk0, int nviews,
+                                                                 const PassK Q)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int v = view_of(blk0, nviews, b);
+    pm_propagate_px<SM>(views[v], Q, (int64_t)(b - blk0[v]) * blockDim.x + threadIdx.x);
+}
+
+template <int SM>
+__global__ void __launch_bounds__(kPMThreads) pm_random_batch(const PMK *views, const int *blk0, int nviews,
+                                                              const PassK Q)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int v = view_of(blk0, nviews, b);
+    pm_random_px<SM>(views[v], Q, (int64_t)(b - blk0[v]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) pm_rank_rows_batch(const PMK *views, const int *row0, int nviews,
+                                                          const PassK Q)
+{
+    const int v = view_of(row0, nviews, blockIdx.x);
+    pm_rank_row(views[v], Q, blockIdx.x - row0[v]);
+}
+
+__global__ void __launch_bounds__(256) pm_scan_rows_batch(const PMK *views, const PassK Q)
+{
+    pm_scan_view(views[blockIdx.x], Q);
+}
+
+__global__ void pm_init_batch(const PMK *views, int nviews)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nviews) {
+        views[v].st[0] = (uint32_t)views[v].seed | 1u;
+        *views[v].changed = 0ull;
+    }
 }
 
 // workspace layout
@@ -544,30 +623,31 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
     const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
     const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
     P.cam = w.cam;
+    P.st = w.st;
     hipLaunchKernelGGL(pm_init_kernel, dim3(1), dim3(1), 0, stream, w.st, seed, w.changed, A, w.cam);
     int pass = 0;
     for (int it = 0; it < iters; ++it) {
-        P.dsc = 0.05 / (it + 1);
-        P.nsc = 0.3 / (it + 1);
         for (int colour = 0; colour < 2; ++colour, ++pass) {
-            P.colour = colour;
-            P.st_in = w.st + (pass & 1);
-            P.st_out = w.st + ((pass + 1) & 1);
+            PassK Q;
+            Q.colour = colour;
+            Q.parity = pass & 1;
+            Q.dsc = 0.05 / (it + 1);
+            Q.nsc = 0.3 / (it + 1);
             {
                 ProfScope ps("pm_propagate", stream);
-                hipLaunchKernelGGL(pm_propagate_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P);
+                hipLaunchKernelGGL(pm_propagate_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
             }
             {
                 ProfScope ps("pm_rank_rows", stream);
-                hipLaunchKernelGGL(pm_rank_rows_kernel, dim3((unsigned)P.H), dim3(256), 0, stream, P);
+                hipLaunchKernelGGL(pm_rank_rows_kernel, dim3((unsigned)P.H), dim3(256), 0, stream, P, Q);
             }
             {
                 ProfScope ps("pm_scan_rows", stream);
-                hipLaunchKernelGGL(pm_scan_rows_kernel, dim3(1), dim3(256), 0, stream, P);
+                hipLaunchKernelGGL(pm_scan_rows_kernel, dim3(1), dim3(256), 0, stream, P, Q);
             }
             {
                 ProfScope ps("pm_random", stream);
-                hipLaunchKernelGGL(pm_random_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P);
+                hipLaunchKernelGGL(pm_random_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
             }
         }
     }
@@ -638,6 +718,7 @@ extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float 
     P.row_cnt = w.row_cnt;
     P.row_off = w.row_off;
     P.changed = w.changed;
+    P.seed = (int)seed;
     P.pinhole = 1;
     PMCamArgs A;
     memset(&A, 0, sizeof A);
@@ -675,4 +756,262 @@ extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float 
         *changed = (int64_t)h;
     }
     return OM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// normals from a depth map (densify._init_normals) and batched PatchMatch (om_batch)
+// ------------------------------------------------------------------------------------------
+#include "batch.h"
+
+namespace om {
+
+// densify.py:481-496 in NumPy's float32 operation order: points = ray * depth, 2-px central
+// differences, np.cross (each product rounded, then subtracted), norm = sqrt((x0^2 + x1^2) + x2^2),
+// (0,0,1) where undefined, flipped to face the camera.
+__device__ __forceinline__ void normal_px(const float *depth, const float *ray, int H, int W, float *normal, int64_t p)
+{
+    const int64_t HW = (int64_t)H * W;
+    const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
+    auto P = [&](int c, int yy, int xx) {
+        const float d = depth[(int64_t)yy * W + xx];
+        const float dn = isfinite(d) ? d : NAN;
+        return ray[c * HW + (int64_t)yy * W + xx] * dn;
+    };
+    float tu[3], tv[3];
+    const bool hx = x >= 2 && x < W - 2, hy = y >= 2 && y < H - 2;
+    for (int c = 0; c < 3; ++c) {
+        tu[c] = hx ? P(c, y, x + 2) - P(c, y, x - 2) : NAN;
+        tv[c] = hy ? P(c, y + 2, x) - P(c, y - 2, x) : NAN;
+    }
+    float n[3];
+    float t;
+    n[0] = tu[1] * tv[2];
+    t = tu[2] * tv[1];
+    n[0] = n[0] - t;
+    n[1] = tu[2] * tv[0];
+    t = tu[0] * tv[2];
+    n[1] = n[1] - t;
+    n[2] = tu[0] * tv[1];
+    t = tu[1] * tv[0];
+    n[2] = n[2] - t;
+    float ss = n[0] * n[0];
+    ss = ss + n[1] * n[1];
+    ss = ss + n[2] * n[2];
+    const float nn = sqrtf(ss);
+    const bool bad = !isfinite(nn) || nn < 1e-12f;
+    if (bad) {
+        n[0] = 0.f;
+        n[1] = 0.f;
+        n[2] = 1.f;
+    } else {
+        n[0] = n[0] / nn;
+        n[1] = n[1] / nn;
+        n[2] = n[2] / nn;
+    }
+    float dot = n[0] * ray[p];
+    dot = dot + n[1] * ray[HW + p];
+    dot = dot + n[2] * ray[2 * HW + p];
+    const float sgn = dot > 0.f ? -1.f : 1.f;
+    for (int c = 0; c < 3; ++c)
+        normal[c * HW + p] = n[c] * sgn;
+}
+
+__global__ void init_normals_kernel(const float *depth, const float *ray, int H, int W, float *normal)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (int64_t)H * W)
+        normal_px(depth, ray, H, W, normal, p);
+}
+
+// batched start of PatchMatch, per pixel of all views: copy the start depth / score maps in
+// (the pass-1 maps stay untouched for the neighbours' geometric term, densify.py:620-634) and
+// optionally set the normals from the start depth (densify._init_normals, densify.py:481-496)
+__global__ void pm_prepare_batch(const PMK *views, const int64_t *px0, int nviews, int normals)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= px0[nviews])
+        return;
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (px0[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const PMK &P = views[lo];
+    const int64_t p = t - px0[lo];
+    if (P.depth_from)
+        P.depth[p] = P.depth_from[p];
+    if (P.score_from)
+        P.score[p] = P.score_from[p];
+    if (normals)
+        normal_px(P.depth_from ? P.depth_from : P.depth, P.ray, P.H, P.W, P.normal, p);
+}
+
+static int sm_bucket(int S) { return S <= 4 ? 4 : (S <= 8 ? 8 : (S <= 12 ? 12 : 16)); }
+
+int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int step, int top_k, double geo_w,
+                 int iters, int init_normals)
+{
+    if (nviews < 1 || !views)
+        return fail(OM_EINVAL, "batch needs at least one view");
+    if (step <= 0 || top_k < 1 || iters < 0)
+        return fail(OM_EINVAL, "pm_refine needs step >= 1, top_k >= 1, iters >= 0");
+    std::vector<PMK> P(nviews);
+    std::vector<int> blk0(nviews + 1, 0), row0(nviews + 1, 0);
+    std::vector<int64_t> px0(nviews + 1, 0);
+    b->pm_changed = (unsigned long long *)b->dev_alloc(sizeof(unsigned long long) * nviews);
+    if (!b->pm_changed)
+        return fail(OM_ENOMEM, "batch: counters");
+    int smax = 1;
+    for (int v = 0; v < nviews; ++v) {
+        const om_pm_view &V = views[v];
+        if (V.S < 1 || V.S > 16)
+            return fail(OM_ENSRC, "view %d: PatchMatch needs 1..16 sources", v);
+        if (V.H < 1 || V.W < 1)
+            return fail(OM_ESHAPE, "view %d: empty image", v);
+        smax = std::max(smax, (int)V.S);
+        char *ws = (char *)b->dev_alloc(pm_ws_layout(V.H, V.W, nullptr, nullptr));
+        if (!ws)
+            return fail(OM_ENOMEM, "batch: PatchMatch workspace");
+        PMWs w;
+        pm_ws_layout(V.H, V.W, ws, &w);
+        PMCam cams[16];
+        memset(cams, 0, sizeof cams);
+        PMK &Q = P[v];
+        memset(&Q, 0, sizeof Q);
+        Q.pinhole = 1;
+        for (int s = 0; s < V.S; ++s) {
+            if (V.ih[s] < 1 || V.iw[s] < 1)
+                return fail(OM_ESHAPE, "view %d source %d has an empty image", v, s);
+            cams[s].img = V.imgs[s];
+            cams[s].dep = V.deps[s];
+            cams[s].ih = V.ih[s];
+            cams[s].iw = V.iw[s];
+            for (int t = 0; t < 18; ++t)
+                cams[s].par[t] = V.par[s][t];
+            if (V.par[s][13] != 0.0 || V.par[s][14] != 0.0 || V.par[s][15] != 0.0)
+                Q.pinhole = 0;
+        }
+        int rc = hip_check(hipMemcpy(w.cam, cams, sizeof(PMCam) * 16, hipMemcpyHostToDevice), "upload");
+        if (rc)
+            return rc;
+        Q.ref = V.ref;
+        Q.ray = V.ray;
+        Q.active = V.active;
+        Q.depth = V.depth;
+        Q.normal = V.normal;
+        Q.score = V.score;
+        Q.H = V.H;
+        Q.W = V.W;
+        Q.S = V.S;
+        Q.r = r;
+        Q.step = step;
+        Q.kk = top_k < V.S ? top_k : V.S;
+        Q.seed = (int)V.seed;
+        Q.cr0 = V.Cr[0];
+        Q.cr1 = V.Cr[1];
+        Q.cr2 = V.Cr[2];
+        Q.geo_w = geo_w;
+        Q.geo_tol = V.geo_tol;
+        Q.dmin = V.dmin;
+        Q.dmax = V.dmax;
+        Q.best = w.best;
+        Q.sc = w.sc;
+        Q.rank = w.rank;
+        Q.row_cnt = w.row_cnt;
+        Q.row_off = w.row_off;
+        Q.st = w.st;
+        Q.changed = b->pm_changed + v;
+        Q.cam = w.cam;
+        Q.depth_from = V.depth_from;
+        Q.score_from = V.score_from;
+        const int64_t ncol = (int64_t)V.H * ((V.W + 1) / 2);
+        blk0[v + 1] = blk0[v] + (int)cdiv(ncol, kPMThreads);
+        row0[v + 1] = row0[v] + V.H;
+        px0[v + 1] = px0[v] + (int64_t)V.H * V.W;
+    }
+    b->pm_views = b->dev_alloc(sizeof(PMK) * nviews);
+    b->pm_blk0 = (int *)b->dev_alloc(sizeof(int) * (nviews + 1));
+    b->pm_row0 = (int *)b->dev_alloc(sizeof(int) * (nviews + 1));
+    b->pm_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
+    if (!b->pm_views || !b->pm_blk0 || !b->pm_row0 || !b->pm_px0)
+        return fail(OM_ENOMEM, "batch: PatchMatch tables");
+    int rc;
+    if ((rc = hip_check(hipMemcpy(b->pm_views, P.data(), sizeof(PMK) * nviews, hipMemcpyHostToDevice), "upload")) ||
+        (rc = hip_check(hipMemcpy(b->pm_blk0, blk0.data(), sizeof(int) * (nviews + 1), hipMemcpyHostToDevice),
+                        "upload")) ||
+        (rc = hip_check(hipMemcpy(b->pm_row0, row0.data(), sizeof(int) * (nviews + 1), hipMemcpyHostToDevice),
+                        "upload")) ||
+        (rc = hip_check(hipMemcpy(b->pm_px0, px0.data(), sizeof(int64_t) * (nviews + 1), hipMemcpyHostToDevice),
+                        "upload")))
+        return rc;
+    b->pm_n = nviews;
+    b->pm_sm = sm_bucket(smax);
+    b->pm_iters = iters;
+    b->pm_blk = blk0[nviews];
+    b->pm_rows = row0[nviews];
+    b->pm_px = px0[nviews];
+    b->pm_normals = init_normals != 0;
+    b->pm_prepare = b->pm_normals;
+    for (int v = 0; v < nviews; ++v)
+        b->pm_prepare = b->pm_prepare || views[v].depth_from || views[v].score_from;
+    return OM_OK;
+}
+
+template <int SM>
+static void pm_batch_passes(om_batch *b, hipStream_t st)
+{
+    const PMK *V = (const PMK *)b->pm_views;
+    int pass = 0;
+    for (int it = 0; it < b->pm_iters; ++it) {
+        for (int colour = 0; colour < 2; ++colour, ++pass) {
+            PassK Q;
+            Q.colour = colour;
+            Q.parity = pass & 1;
+            Q.dsc = 0.05 / (it + 1);
+            Q.nsc = 0.3 / (it + 1);
+            hipLaunchKernelGGL(pm_propagate_batch<SM>, dim3((unsigned)b->pm_blk), dim3(kPMThreads), 0, st, V,
+                               b->pm_blk0, b->pm_n, Q);
+            hipLaunchKernelGGL(pm_rank_rows_batch, dim3((unsigned)b->pm_rows), dim3(256), 0, st, V, b->pm_row0,
+                               b->pm_n, Q);
+            hipLaunchKernelGGL(pm_scan_rows_batch, dim3((unsigned)b->pm_n), dim3(256), 0, st, V, Q);
+            hipLaunchKernelGGL(pm_random_batch<SM>, dim3((unsigned)b->pm_blk), dim3(kPMThreads), 0, st, V,
+                               b->pm_blk0, b->pm_n, Q);
+        }
+    }
+}
+
+int batch_pm_enqueue(om_batch *b, hipStream_t st)
+{
+    if (b->pm_n == 0)
+        return OM_OK;
+    ProfScope ps("pm_batch", st);
+    const PMK *V = (const PMK *)b->pm_views;
+    hipLaunchKernelGGL(pm_init_batch, dim3((unsigned)cdiv(b->pm_n, 64)), dim3(64), 0, st, V, b->pm_n);
+    if (b->pm_prepare)
+        hipLaunchKernelGGL(pm_prepare_batch, dim3((unsigned)cdiv(b->pm_px, 256)), dim3(256), 0, st, V, b->pm_px0,
+                           b->pm_n, (int)b->pm_normals);
+    switch (b->pm_sm) {
+    case 4: pm_batch_passes<4>(b, st); break;
+    case 8: pm_batch_passes<8>(b, st); break;
+    case 12: pm_batch_passes<12>(b, st); break;
+    default: pm_batch_passes<16>(b, st); break;
+    }
+    return launch_check("pm batch kernels");
+}
+
+}  // namespace om
+
+extern "C" int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream)
+{
+    if (H < 0 || W < 0)
+        return fail(OM_ESHAPE, "negative size");
+    const int64_t n = (int64_t)H * W;
+    if (n == 0)
+        return OM_OK;
+    hipLaunchKernelGGL(init_normals_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, as_stream(stream), depth, ray,
+                       H, W, normal);
+    return launch_check("init_normals_kernel");
 }

@@ -44,8 +44,10 @@ constexpr int kRunsPerRow = kTW / kRun;           // 8
 constexpr int kHRows = kThreads / kRunsPerRow;    // 32 halo rows per horizontal pass
 constexpr int kVSeg = kThreads / kTW;             // 4 row segments per column in the vertical pass
 constexpr int kMaxR = 7;                          // largest window radius of the fused kernel
-constexpr int kVR = 4;                            // output rows per thread in the vertical pass
-constexpr int kTH = kVSeg * kVR;                  // 16 output rows per tile
+// Tile heights: 4 * VR output rows, VR = output rows per thread in the vertical pass.  VR = 4
+// (16-row tiles, less halo) is the default; VR = 2 (8-row tiles: less LDS and fewer registers,
+// more workgroups) is the alternative the autotuner can pick for small images.
+constexpr int kVRDefault = 4;
 constexpr int kPartPlanes = 6;                    // chunk partials: best, idx, prev-of-best, next-of-best, first, last
 
 struct SrcK {
@@ -66,6 +68,7 @@ struct SweepK {
     float *best, *prev, *spb, *snb;
     int *idx;
     float *part;              // (chunks, 6, H, W) when chunks > 1
+    float *depth_out;         // batched launches: parabola depth of the view, or null
     SrcK src[16];
 };
 
@@ -103,12 +106,13 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
     }
 }
 
-template <int K>
-__global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
+// One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
+// device memory (batched launches).
+template <int K, int VR>
+__device__ __forceinline__ void sweep_tile(const SweepK &P, int tile, int chunk, double *lds_d)
 {
-    extern __shared__ double lds_d[];
     const int r = P.r;
-    constexpr int TH = kTH;
+    constexpr int TH = kVSeg * VR;
     const int HR = TH + 2 * r;          // halo rows
     const int HC = kTW + 2 * r;         // halo columns
     const int NHALO = HR * HC;
@@ -119,10 +123,6 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
     float *sH = sJ + NHALO;                      // (4, HR, kTW) horizontal sums
 
     const int tid = threadIdx.x;
-    const int ntiles = P.tiles_x * P.tiles_y;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
-    const int tile = lb / P.chunks;
-    const int chunk = lb % P.chunks;
     const int tx0 = (tile % P.tiles_x) * kTW;
     const int ty0 = (tile / P.tiles_x) * TH;
     const int ia = P.i0 + chunk * P.hc;
@@ -141,16 +141,16 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
         sI[hp] = in ? P.ref[p] : 0.f;
     }
 
-    // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+kVR-1
-    const int vc = tid % kTW, vseg = tid / kTW, vy0 = vseg * kVR;
+    // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+VR-1
+    const int vc = tid % kTW, vseg = tid / kTW, vy0 = vseg * VR;
     const int ox = tx0 + vc;
-    float mu_[kVR], sd_[kVR];
-    float st_best[kVR], st_prev[kVR], st_spb[kVR], st_snb[kVR], st_first[kVR];
-    int st_idx[kVR];
-    float tk[kVR][K];
+    float mu_[VR], sd_[VR];
+    float st_best[VR], st_prev[VR], st_spb[VR], st_snb[VR], st_first[VR];
+    int st_idx[VR];
+    float tk[VR][K];
     const bool chunked = P.chunks > 1;
 #pragma unroll
-    for (int j = 0; j < kVR; ++j) {
+    for (int j = 0; j < VR; ++j) {
         const int oy = ty0 + vy0 + j;
         const bool in = oy < H && ox < W;
         const size_t p = in ? (size_t)oy * W + ox : 0;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
             sD[hp] = d;
         }
 #pragma unroll
-        for (int j = 0; j < kVR; ++j)
+        for (int j = 0; j < VR; ++j)
 #pragma unroll
             for (int a = 0; a < K; ++a)
                 tk[j][a] = -INFINITY;
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
                     c3 = c3 + (double)h3[o];
                 }
 #pragma unroll
-                for (int j = 0; j < kVR; ++j) {
+                for (int j = 0; j < VR; ++j) {
                     {
                         if (j > 0) {
                             const int on = (vy0 + j + 2 * r) * kTW, oo = (vy0 + j - 1) * kTW;
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
         }
         // ---- combine + streaming winner (_sweep.h:181-214)
 #pragma unroll
-        for (int j = 0; j < kVR; ++j) {
+        for (int j = 0; j < VR; ++j) {
             {
                 float s = 0.0f;
 #pragma unroll
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
     }
 
 #pragma unroll
-    for (int j = 0; j < kVR; ++j) {
+    for (int j = 0; j < VR; ++j) {
         const int oy = ty0 + vy0 + j;
         if (oy < H && ox < W) {
             const size_t p = (size_t)oy * W + ox;
@@ -367,13 +367,49 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
     }
 }
 
-// Replays the streaming winner rule (_sweep.h:204-212) over the chunks in hypothesis order.
-__global__ void sweep_merge_kernel(const float *__restrict__ part, int chunks, int64_t HW, float *best, float *prev,
-                                   float *spb, float *snb, int *idx, int i0, int hc)
+template <int K, int VR>
+__global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= HW)
-        return;
+    extern __shared__ double lds_d[];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
+    sweep_tile<K, VR>(P, lb / P.chunks, lb % P.chunks, lds_d);
+}
+
+// Persistent batch: work items (view, tile, chunk) of many views from one queue.  Every wave of a
+// workgroup reads the same item index from LDS, so all of them leave the loop together once the
+// queue is empty.
+template <int K, int VR>
+__global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *views, const int *item0, int nviews,
+                                                               int total, int *counter)
+{
+    extern __shared__ double lds_d[];
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= total)
+            break;
+        int lo = 0, hi = nviews - 1;  // last view whose first item is <= it
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (item0[mid] <= it)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const SweepK &P = views[lo];
+        const int local = it - item0[lo];
+        sweep_tile<K, VR>(P, local / P.chunks, local % P.chunks, lds_d);
+    }
+}
+
+// Replays the streaming winner rule (_sweep.h:204-212) over the chunks in hypothesis order.
+__device__ __forceinline__ void merge_px(const float *__restrict__ part, int chunks, int64_t HW, float *best,
+                                         float *prev, float *spb, float *snb, int *idx, int i0, int hc, int64_t p)
+{
     float gb = best[p], gp = prev[p], gs = spb[p], gn = snb[p];
     int gi = idx[p];
     for (int c = 0; c < chunks; ++c) {
@@ -401,18 +437,22 @@ __global__ void sweep_merge_kernel(const float *__restrict__ part, int chunks, i
     idx[p] = gi;
 }
 
-// parabola of densify._sweep (densify.py:354-361), NumPy's float32 / float64 promotion kept
-__global__ void sweep_finalize_kernel(SweepK P, float *depth)
+__global__ void sweep_merge_kernel(const float *__restrict__ part, int chunks, int64_t HW, float *best, float *prev,
+                                   float *spb, float *snb, int *idx, int i0, int hc)
 {
-    const int64_t HW = (int64_t)P.H * P.W;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= HW)
-        return;
+    if (p < HW)
+        merge_px(part, chunks, HW, best, prev, spb, snb, idx, i0, hc, p);
+}
+
+// parabola of densify._sweep (densify.py:354-361), NumPy's float32 / float64 promotion kept
+__device__ __forceinline__ void finalize_px(const SweepK &P, int64_t p, float *depth)
+{
     const float a = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.ha[p] : 0.f;
     const float b = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.hb[p] : 0.f;
-    const float inv0 = hyp_value(P, 0, (int)p, a, b);
+    const float inv0 = hyp_value(P, P.i0, (int)p, a, b);
     const int n = P.nh;
-    const float dinv = n > 1 ? hyp_value(P, 1, (int)p, a, b) - inv0 : 0.0f;
+    const float dinv = n > 1 ? hyp_value(P, P.i0 + 1, (int)p, a, b) - inv0 : 0.0f;
     const float best = P.best[p], spb = P.spb[p], snb = P.snb[p];
     const int idx = P.idx[p];
     const float den = (spb - 2.0f * best) + snb;
@@ -424,6 +464,62 @@ __global__ void sweep_finalize_kernel(SweepK P, float *depth)
     depth[p] = (float)(1.0 / m);
 }
 
+__global__ void sweep_finalize_kernel(SweepK P, float *depth)
+{
+    const int64_t HW = (int64_t)P.H * P.W;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < HW)
+        finalize_px(P, p, depth);
+}
+
+// batched merge + parabola over the pixels of all views: view v owns [px0[v], px0[v+1])
+__device__ __forceinline__ int view_of_px(const int64_t *px0, int nviews, int64_t t)
+{
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (px0[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void sweep_batch_post_kernel(const SweepK *views, const int64_t *px0, int nviews, int do_merge,
+                                        int do_final)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= px0[nviews])
+        return;
+    const int v = view_of_px(px0, nviews, t);
+    const SweepK &P = views[v];
+    const int64_t p = t - px0[v];
+    const int64_t HW = (int64_t)P.H * P.W;
+    if (do_merge && P.chunks > 1)
+        merge_px(P.part, P.chunks, HW, P.best, P.prev, P.spb, P.snb, P.idx, P.i0, P.hc, p);
+    if (do_final && P.depth_out)
+        finalize_px(P, p, P.depth_out);
+}
+
+__global__ void reset_counter_kernel(int *c) { *c = 0; }
+
+// a batch run starts every view from the initial winner state of densify._sweep
+// (densify.py:331-335): best = prev = s_prev_best = s_next_best = -2, idx = -1
+__global__ void sweep_batch_reset_kernel(const SweepK *views, const int64_t *px0, int nviews)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= px0[nviews])
+        return;
+    const int v = view_of_px(px0, nviews, t);
+    const SweepK &P = views[v];
+    const int64_t p = t - px0[v];
+    P.best[p] = -2.0f;
+    P.prev[p] = -2.0f;
+    P.spb[p] = -2.0f;
+    P.snb[p] = -2.0f;
+    P.idx[p] = -1;
+}
 
 static int build_params(const om_sweep_desc *d, SweepK &P)
 {
@@ -478,9 +574,9 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
         if (o.geo[13] != 0.0 || o.geo[14] != 0.0 || o.geo[15] != 0.0)
             P.pinhole = 0;
     }
-    if (d->tile_h > 0 && d->tile_h != kTH)
-        return fail(OM_EINVAL, "tile height is fixed at %d rows (got %d)", kTH, d->tile_h);
-    P.TH = kTH;
+    if (d->tile_h != 0 && d->tile_h != 8 && d->tile_h != 16)
+        return fail(OM_EINVAL, "tile height must be 8 or 16 rows (got %d)", d->tile_h);
+    P.TH = d->tile_h ? d->tile_h : kVSeg * kVRDefault;
     P.tiles_x = (int)cdiv(d->W, kTW);
     P.tiles_y = (int)cdiv(d->H, P.TH);
     return OM_OK;
@@ -542,8 +638,9 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
     hipStream_t st = as_stream(stream);
     ProfScope ps("sweep", st);
     using KFn = void (*)(const SweepK);
-    static const KFn table[4] = {sweep_kernel<1>, sweep_kernel<2>, sweep_kernel<3>, sweep_kernel<4>};
-    hipLaunchKernelGGL(table[Pl.k - 1], grid, dim3(kThreads), lds, st, Pl);
+    static const KFn table[4][2] = {{sweep_kernel<1, 2>, sweep_kernel<1, 4>}, {sweep_kernel<2, 2>, sweep_kernel<2, 4>},
+                                    {sweep_kernel<3, 2>, sweep_kernel<3, 4>}, {sweep_kernel<4, 2>, sweep_kernel<4, 4>}};
+    hipLaunchKernelGGL(table[Pl.k - 1][Pl.TH == 16], grid, dim3(kThreads), lds, st, Pl);
     if ((rc = launch_check("sweep_kernel")))
         return rc;
     if (Pl.chunks > 1) {
@@ -616,3 +713,134 @@ extern "C" int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *str
                        depth);
     return launch_check("sweep_finalize_kernel");
 }
+
+// ------------------------------------------------------------------------------------------
+// batched sweep (om_batch): descriptors of all views in device memory, one persistent launch
+// ------------------------------------------------------------------------------------------
+#include "batch.h"
+
+namespace om {
+
+using BatchFn = void (*)(const SweepK *, const int *, int, int, int *);
+
+static BatchFn batch_kernel(int k, int th)
+{
+    static const BatchFn table[4][2] = {
+        {sweep_batch_kernel<1, 2>, sweep_batch_kernel<1, 4>}, {sweep_batch_kernel<2, 2>, sweep_batch_kernel<2, 4>},
+        {sweep_batch_kernel<3, 2>, sweep_batch_kernel<3, 4>}, {sweep_batch_kernel<4, 2>, sweep_batch_kernel<4, 4>}};
+    return table[k - 1][th == 16];
+}
+
+static size_t sweep_lds_bytes(int th, int r)
+{
+    const size_t halo = (size_t)(th + 2 * r) * (kTW + 2 * r);
+    return halo * (sizeof(double) + sizeof(int) + 2 * sizeof(float)) + sizeof(float) * 4 * (size_t)(th + 2 * r) * kTW;
+}
+
+int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *const *depth_out, int chunks,
+                    int tile_h, int grid_blocks)
+{
+    if (nviews < 1 || !views)
+        return fail(OM_EINVAL, "batch needs at least one view");
+    std::vector<SweepK> P(nviews);
+    int k = 0, r = -1;
+    int64_t tiles = 0;
+    for (int v = 0; v < nviews; ++v) {
+        om_sweep_desc d = views[v];
+        d.tile_h = tile_h;
+        int rc = build_params(&d, P[v]);
+        if (rc)
+            return fail(rc, "view %d: %s", v, om_last_error());
+        if (v == 0) {
+            k = P[v].k;
+            r = P[v].r;
+        } else if (P[v].k != k || P[v].r != r) {
+            return fail(OM_EINVAL, "all views of a batch need the same window radius and min(top_k, n)");
+        }
+        if (d.i0 != 0)  // every run starts each view from the initial winner state
+            return fail(OM_EINVAL, "view %d: a batch sweeps hypotheses from i0 = 0", v);
+        tiles += (int64_t)P[v].tiles_x * P[v].tiles_y;
+    }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    int c = chunks;
+    if (c <= 0)
+        c = (int)std::min<int64_t>(std::max<int64_t>(cdiv(4LL * cus, std::max<int64_t>(tiles, 1)), 1), 64);
+    std::vector<int> item0(nviews);
+    std::vector<int64_t> px0(nviews + 1, 0);
+    int64_t total = 0;
+    bool merge = false, final_ = false;
+    for (int v = 0; v < nviews; ++v) {
+        SweepK &Q = P[v];
+        const int cv = std::max(1, std::min(c, std::max(Q.nh, 1)));
+        Q.hc = (int)cdiv(std::max(Q.nh, 1), cv);
+        Q.chunks = (int)cdiv(std::max(Q.nh, 1), Q.hc);
+        const int64_t HW = (int64_t)Q.H * Q.W;
+        if (Q.chunks > 1) {
+            Q.part = (float *)b->dev_alloc((size_t)Q.chunks * kPartPlanes * HW * sizeof(float));
+            if (!Q.part)
+                return fail(OM_ENOMEM, "batch: chunk partials");
+            merge = true;
+        }
+        Q.depth_out = depth_out ? depth_out[v] : nullptr;
+        final_ = final_ || Q.depth_out;
+        item0[v] = (int)total;
+        total += (int64_t)Q.tiles_x * Q.tiles_y * Q.chunks;
+        px0[v + 1] = px0[v] + HW;
+    }
+    if (total >= INT_MAX)
+        return fail(OM_EUNSUP, "batch has too many work items");
+    b->sw_views = b->dev_alloc(sizeof(SweepK) * nviews);
+    b->sw_item0 = (int *)b->dev_alloc(sizeof(int) * nviews);
+    b->sw_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
+    b->sw_counter = (int *)b->dev_alloc(sizeof(int));
+    if (!b->sw_views || !b->sw_item0 || !b->sw_px0 || !b->sw_counter)
+        return fail(OM_ENOMEM, "batch: descriptor tables");
+    int rc;
+    if ((rc = hip_check(hipMemcpy(b->sw_views, P.data(), sizeof(SweepK) * nviews, hipMemcpyHostToDevice), "upload")) ||
+        (rc = hip_check(hipMemcpy(b->sw_item0, item0.data(), sizeof(int) * nviews, hipMemcpyHostToDevice), "upload")) ||
+        (rc = hip_check(hipMemcpy(b->sw_px0, px0.data(), sizeof(int64_t) * (nviews + 1), hipMemcpyHostToDevice),
+                        "upload")))
+        return rc;
+    b->sw_n = nviews;
+    b->sw_k = k;
+    b->sw_th = P[0].TH;
+    b->sw_total = (int)total;
+    b->sw_px = px0[nviews];
+    b->sw_merge = merge;
+    b->sw_final = final_;
+    b->sw_lds = sweep_lds_bytes(b->sw_th, r);
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, b->sw_th), kThreads,
+                                                     b->sw_lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const int resident = per_cu * cus;
+    b->sw_grid = grid_blocks > 0 ? grid_blocks : (int)std::min<int64_t>(resident, total);
+    return OM_OK;
+}
+
+int batch_sweep_enqueue(om_batch *b, hipStream_t st)
+{
+    if (b->sw_n == 0)
+        return OM_OK;
+    ProfScope ps("sweep_batch", st);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, b->sw_counter);
+    hipLaunchKernelGGL(sweep_batch_reset_kernel, dim3((unsigned)cdiv(b->sw_px, 256)), dim3(256), 0, st,
+                       (const SweepK *)b->sw_views, b->sw_px0, b->sw_n);
+    hipLaunchKernelGGL(batch_kernel(b->sw_k, b->sw_th), dim3((unsigned)b->sw_grid), dim3(kThreads), b->sw_lds, st,
+                       (const SweepK *)b->sw_views, b->sw_item0, b->sw_n, b->sw_total, b->sw_counter);
+    int rc = launch_check("sweep_batch_kernel");
+    if (rc)
+        return rc;
+    if (b->sw_merge || b->sw_final) {
+        hipLaunchKernelGGL(sweep_batch_post_kernel, dim3((unsigned)cdiv(b->sw_px, 256)), dim3(256), 0, st,
+                           (const SweepK *)b->sw_views, b->sw_px0, b->sw_n, (int)b->sw_merge, (int)b->sw_final);
+        rc = launch_check("sweep_batch_post_kernel");
+    }
+    return rc;
+}
+
+}  // namespace om

@@ -210,18 +210,6 @@ def depth_map(k, nbrs, rec, views: Callable, opt: DepthOptions, device="cuda"):
 
 def init_normals(D: torch.Tensor, ray: torch.Tensor) -> torch.Tensor:
     """World normals of a depth map from 2-px central differences, facing the camera, (0,0,1) where
-    undefined (densify.py:481-496).  -> (3, H, W) float32."""
-    P = ray * torch.where(torch.isfinite(D), D, torch.full_like(D, float("nan")))[None]
-    tu = torch.full_like(P, float("nan"))
-    tv = torch.full_like(P, float("nan"))
-    tu[:, :, 2:-2] = P[:, :, 4:] - P[:, :, :-4]
-    tv[:, 2:-2, :] = P[:, 4:, :] - P[:, :-4, :]
-    n = torch.cross(tu, tv, dim=0)
-    nn = torch.linalg.norm(n, dim=0)
-    bad = ~torch.isfinite(nn) | (nn < 1e-12)
-    n = n / torch.where(bad, torch.ones_like(nn), nn)[None]
-    up = torch.tensor([0.0, 0.0, 1.0], dtype=n.dtype, device=n.device)[:, None]
-    n[:, bad] = up.expand(3, int(bad.sum()))
-    flip = (n * ray).sum(0) > 0
-    n[:, flip] *= -1
-    return n.to(torch.float32).contiguous()
+    undefined (densify.py:481-496), in NumPy's float32 operation order.  -> (3, H, W) float32."""
+    from .batch import init_normals as _k
+    return _k(_dev(D, D.device), _dev(ray, ray.device))
