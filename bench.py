@@ -271,15 +271,221 @@ def run_c1(args, torch):
     return res
 
 
+# ------------------------------------------------------------------------------- C3 batched block
+def _survey_c3(torch, cache, scale):
+    """The C3 block, rendered on the device or loaded from `cache` (written on first use)."""
+    from py_ortho_amd import scene
+    sizes = [(int(round(w * scale)), int(round(h * scale))) for w, h in scene.C3_SIZES]
+    cams, counts, ext = scene.survey_cameras(4, 8, 6, 90.0, sizes, 0.9375, 0.7, 0.6, (4, 12))
+    if cache and os.path.exists(cache):
+        z = np.load(cache, allow_pickle=False)
+        views = [scene.View(c, torch.from_numpy(z[f"img{k}"]).cuda(), torch.from_numpy(z[f"dep{k}"]).cuda())
+                 for k, c in enumerate(cams)]
+        C = np.stack([c.C for c in cams])
+        return scene.Survey(scene.make_terrain(4, ext, n_bumps=40), views,
+                            scene.neighbours_by_distance(C, 90.0, counts), 90.0 - 17.0, 92.0)
+    sv = scene.config_c3(device="cuda", scale=scale)
+    if cache:
+        np.savez(cache, **{f"img{k}": v.img.cpu().numpy() for k, v in enumerate(sv.views)},
+                 **{f"dep{k}": v.depth.cpu().numpy() for k, v in enumerate(sv.views)})
+    return sv
+
+
+def _depth_accuracy(maps, truths):
+    """Over all pixels of all views: fraction within 1 % of the rendered depth, median relative error
+    of the finite ones, fraction finite."""
+    import torch
+    within, errs = 0.0, []
+    for d, t in zip(maps, truths):
+        rel = ((d - t).abs() / t).float()
+        rel = rel[torch.isfinite(rel)]
+        within += float((rel < 0.01).float().sum())
+        errs.append(rel)
+    allr = torch.cat(errs)
+    n = sum(int(t.numel()) for t in truths)
+    return {"frac_within_1pct": within / n, "median_rel_err": float(allr.median()),
+            "frac_finite": float(allr.numel()) / n}
+
+
+def run_c3(args, torch):
+    from py_ortho_amd import _lib, autotune, block
+    lib = _lib.lib()
+    t_scene = time.perf_counter()
+    sv = _survey_c3(torch, args.scene_cache, args.scale)
+    torch.cuda.synchronize()
+    t_scene = time.perf_counter() - t_scene
+    n = len(sv.views)
+    blk = block.BlockDepth([v.img.contiguous() for v in sv.views], [v.cam for v in sv.views], sv.nbrs,
+                           [(sv.dmin, sv.dmax)] * n)
+    o = blk.opt
+    sw_ph, pm_ph = blk.pixel_hypotheses()
+    t_tune = time.perf_counter()
+    tune, table = autotune.tune_block(blk)
+    t_tune = time.perf_counter() - t_tune
+    blk.plan(tune)
+    blk.run()  # warm-up, eager
+    torch.cuda.synchronize()
+    # one eager run with HIP events around the dominant kernels
+    lib.om_profile_enable(1)
+    prof_read(lib, "sweep_batch_kernel")
+    blk.run()
+    torch.cuda.synchronize()
+    lib.om_profile_enable(0)
+    sw_ms, sw_n = prof_read(lib, "sweep_batch_kernel")
+    pr_ms, pr_n = prof_read(lib, "pm_propagate_batch")
+    rn_ms, rn_n = prof_read(lib, "pm_random_batch")
+    lib.om_profile_read(None, None, None)
+    # the timed steps: graph replays of the whole block (sweep, parabola, PatchMatch of all views)
+    blk.capture()
+    for _ in range(args.warmup):
+        blk.replay()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        blk.replay()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    ms = wall / args.steps * 1e3
+    raw_b = [d.clone() for d in blk.raw]
+    dep_b = [d.clone() for d in blk.depth]
+    truths = [v.depth for v in sv.views]
+    acc = {"sweep": _depth_accuracy(raw_b, truths), "patchmatch": _depth_accuracy(dep_b, truths)}
+    # the plain per-view form: fused single-view sweep + parabola + single-view PatchMatch per view
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    ts = time.perf_counter()
+    blk.run_single()
+    torch.cuda.synchronize()
+    single_ms = (time.perf_counter() - ts) * 1e3
+    lib.om_profile_enable(0)
+    single_parts = {name: prof_read(lib, name)[0] for name in ("sweep", "pm_propagate", "pm_random")}
+    lib.om_profile_read(None, None, None)
+    same = all(bool(torch.equal(a.view(torch.int32), b.view(torch.int32))) for a, b in zip(raw_b, blk.raw)) and \
+        all(bool(torch.equal(a.view(torch.int32), b.view(torch.int32))) for a, b in zip(dep_b, blk.depth))
+    plain = {"fused_per_view_ms": single_ms, "speedup_vs_fused_per_view": single_ms / ms,
+             "fused_per_view_kernel_ms": single_parts}
+    if args.plain_per_hyp:
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        blk.run_single(per_hypothesis=True)
+        torch.cuda.synchronize()
+        ph_ms = (time.perf_counter() - ts) * 1e3
+        plain.update({"per_hypothesis_ms": ph_ms, "speedup_vs_per_hypothesis": ph_ms / ms})
+    flops = sum(px * o.num_depths * sweep_flops_per_hyp(len(nb)) for px, nb in zip(blk.pixels(), blk.nbrs))
+    sw_avg = sw_ms / max(sw_n, 1) / 1e3
+    sw_tot = sw_ms / 1e3
+    achieved = flops / sw_tot if sw_tot > 0 else 0.0
+    S_list = [len(nb) for nb in blk.nbrs]
+    res = {
+        "metric": METRIC, "value": (sw_ph + pm_ph) / (ms / 1e3) / 1e6, "unit": "Mpixel-hyp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"C3 batched block: {n} views (8x6 grid, 90 m, 70/60 pct overlap) of sizes "
+                               f"800x600..2400x1800 x {args.scale:g}, S=U{{4..12}}, {o.num_depths} planes 11x11 NCC "
+                               f"top-2 + parabola, then {o.pm_iters} PatchMatch iterations (9x9 window step 2) on "
+                               "every pixel, one HIP graph",
+                   "views": n, "megapixels": sum(blk.pixels()) / 1e6, "sources_mean": float(np.mean(S_list)),
+                   "sweep_pixel_hypotheses": sw_ph, "pm_pixel_hypotheses": pm_ph,
+                   "sweep_mpix_hyp_s": sw_ph / sw_tot / 1e6 if sw_tot else None,
+                   "pm_mpix_hyp_s": pm_ph / ((pr_ms + rn_ms) / 1e3) / 1e6 if pr_ms + rn_ms > 0 else None,
+                   "autotune": {f"{h}x{w}": list(v) for (h, w), v in tune.items()},
+                   "autotune_ns_per_pixel_hyp": {f"{h}x{w}": {f"th{a}_c{b}": round(t, 4) for (a, b), t in d.items()}
+                                                 for (h, w), d in table.items()},
+                   "autotune_s": t_tune, "scene_s": t_scene},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
+                     "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_batch_kernel",
+                     "kernel_avg_ms": sw_avg * 1e3, "launches": sw_n, "flops_per_step": flops,
+                     "pm_propagate_ms": pr_ms, "pm_random_ms": rn_ms},
+        "plain_single_view": plain, "batch_equals_single_view": same,
+        "accuracy": acc,
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_c3(blk, sv, torch, args)
+    return res
+
+
+def cpu_baseline_c3(blk, sv, torch, args):
+    """The oracle's sweep (linear planes, same count) and pm_refine on a centre crop of the view with
+    the median source count, `cores` concurrent copies each; Mpixel-hyp/s = cores * (D + 11 iters) /
+    (sweep + PatchMatch wall time per crop pixel)."""
+    import oracle
+    o = blk.opt
+    cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
+    S_list = [len(nb) for nb in blk.nbrs]
+    k = int(np.argsort(S_list)[len(S_list) // 2])
+    H, W = blk.size_class(k)
+    ref = blk.ref[k].cpu().numpy()
+    mu, sd = (t.cpu().numpy() for t in blk.stats[k])
+    ray = blk.rays[k].cpu().numpy()
+    srcs = [blk.imgs[j].cpu().numpy() for j in blk.nbrs[k]]
+    deps = [blk.raw[j].cpu().numpy() for j in blk.nbrs[k]]
+    geo = np.stack([block_geo(blk, j, k) for j in blk.nbrs[k]])
+    par = blk.pm_view(k).par
+    planes = blk.planes_host[k]
+    raw = blk.raw[k].cpu().numpy()
+    nrm = blk.normal[k].cpu().numpy()
+
+    def crop_sl(side):
+        y0, x0 = (H - side) // 2, (W - side) // 2
+        return y0, x0, np.s_[y0:y0 + side, x0:x0 + side]
+
+    def sweep(side, nd, threads):
+        y0, x0, sl = crop_sl(side)
+        base = np.full((side, side), planes[0], np.float32)
+        step = np.full((side, side), planes[1] - planes[0], np.float32)
+        t = time.perf_counter()
+        oracle.bench_sweep(threads, ref[sl], mu[sl], sd[sl], srcs, np.ascontiguousarray(ray[:, y0:y0 + side,
+                                                                                         x0:x0 + side]),
+                           geo, base, step, nd, o.window, o.top_k)
+        return time.perf_counter() - t
+
+    def pm(side, iters, threads):
+        y0, x0, sl = crop_sl(side)
+        a = np.ones((side, side), np.uint8)
+        t = time.perf_counter()
+        oracle.bench_pm(threads, ref[sl], np.ascontiguousarray(ray[:, y0:y0 + side, x0:x0 + side]),
+                        np.asarray(blk.cams[k].C, np.float64), srcs, deps, par, np.ascontiguousarray(raw[sl]),
+                        np.ascontiguousarray(nrm[:, y0:y0 + side, x0:x0 + side]), a, o.pm_window, o.pm_step, o.top_k,
+                        o.pm_geo_weight, o.geo_tol, iters, sv.dmin, sv.dmax, 1)
+        return time.perf_counter() - t
+
+    target = 1.5                                     # seconds of work per core for each part
+    sw_px = sweep(16, 8, 1) / (256 * 8)              # seconds per pixel-plane, one core
+    side_s = int(max(16, min(512, math.sqrt(target / (sw_px * o.num_depths)))))
+    pm_px = pm(16, 1, 1) / 256                       # seconds per pixel-iteration, one core
+    side_p = int(max(16, min(512, math.sqrt(target / (pm_px * o.pm_iters)))))
+    ws = sweep(side_s, o.num_depths, cores)
+    wp = pm(side_p, o.pm_iters, cores)
+    per_px = ws / (side_s * side_s) + wp / (side_p * side_p)   # wall seconds per pixel for `cores` copies
+    return {"value": cores * (o.num_depths + 11 * o.pm_iters) / per_px / 1e6, "unit": "Mpixel-hyp/s",
+            "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent copies of the oracle sweep ({o.num_depths} planes, {side_s}x{side_s} "
+                      f"centre crop) and pm_refine ({o.pm_iters} iterations, {side_p}x{side_p} crop) of view {k} "
+                      f"({W}x{H}, S={S_list[k]}); {ws:.1f} s + {wp:.1f} s wall"}
+
+
+def block_geo(blk, j, k):
+    from py_ortho_amd.block import _geo18
+    return _geo18(blk.cams[j], blk.cams[k])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3"], default="c2")
+    ap.add_argument("--scale", type=float, default=1.0, help="C3: image size scale (1 = the full block)")
+    ap.add_argument("--plain-per-hyp", action="store_true",
+                    help="C3: also time the per-view form with one sweep launch per plane")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = all cores)")
-    ap.add_argument("--scene-cache", default=None, help="npz file holding the rendered C2 scene (created if absent)")
+    ap.add_argument("--scene-cache", default=None, help="npz file holding the rendered scene (created if absent)")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first run one plain elementwise kernel over this many MiB (read N, write N) as the "
                          "reference point for hardware byte counters")
@@ -299,7 +505,7 @@ def main(argv=None):
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    res = run_c2(args, torch) if args.workload == "c2" else run_c1(args, torch)
+    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3}[args.workload](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

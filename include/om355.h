@@ -160,11 +160,12 @@ typedef struct om_pm_view {
 
 om_batch *om_batch_create(void);
 int om_batch_destroy(om_batch *b);
-/* sweep of nviews views sharing r and min(top_k, n_src) (each desc's workspace/chunks ignored);
+/* sweep of nviews views sharing r and min(top_k, n_src) (each desc's workspace is ignored);
  * every run resets each view's winner state and sweeps its hypotheses [0, nh) (i0 must be 0);
  * depth_out[v] (array may be NULL, entries may be NULL) receives view v's parabola depth;
- * chunks: hypothesis chunks per view (0 = auto); tile_h: 8 or 16 (0 = 16); grid_blocks: persistent
- * workgroups (0 = all resident). */
+ * chunks: hypothesis chunks for every view (0 = each desc's chunks, 0 there = auto);
+ * tile_h: 8 or 16 for every view (0 = each desc's tile_h, 0 there = 16); views of one tile height
+ * share one persistent launch; grid_blocks: workgroups per launch (0 = all resident). */
 int om_batch_set_sweep(om_batch *b, const om_sweep_desc *views, int32_t nviews, float *const *depth_out,
                        int32_t chunks, int32_t tile_h, int32_t grid_blocks);
 /* PatchMatch over nviews views after the sweep (same r, step, top_k, geo_w, iters for all);

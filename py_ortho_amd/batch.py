@@ -32,6 +32,8 @@ class SweepView:
     hyp: dict
     state: Optional[dense.SweepState] = None
     depth: Optional[torch.Tensor] = None      # receives the parabola depth when given
+    tile_h: int = 0                           # 8 or 16 rows (0 = 16) unless the batch sets one for all
+    chunks: int = 0                           # hypothesis chunks (0 = auto) unless the batch sets one for all
 
 
 @dataclass
@@ -90,7 +92,7 @@ class Batch:
                 V.state = dense.SweepState(H, W, V.ref.device)
             mode, a, b, off, n_all = dense._hyp_inputs(V.hyp, H, W)
             d, keep = dense._sweep_desc(V.ref, V.mu, V.sd, list(V.srcs), V.ray, V.geo, r, top_k, V.state, mode, a, b,
-                                        off, 0, n_all)
+                                        off, 0, n_all, int(V.chunks), int(V.tile_h))
             descs[v] = d
             self._keep += [keep, V]
             outs[v] = V.depth.data_ptr() if V.depth is not None else None

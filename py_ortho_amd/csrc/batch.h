@@ -4,27 +4,42 @@
 
 #include "om_common.h"
 
+// one persistent sweep launch: the views of one tile height
+struct SweepGroup {
+    int th = 16, first = 0, nv = 0, total = 0, grid = 0;
+    size_t lds = 0;
+};
+
+// the PatchMatch views of one source bucket: table entries [first, first + nv)
+struct PMGroup {
+    int sm = 16, first = 0, nv = 0, blk = 0, blk0_off = 0;
+};
+
 struct om_batch {
     std::vector<void *> allocs;  // device buffers owned by the batch
 
-    // plane sweep over all views: one persistent launch + one post-processing launch
-    int sw_n = 0, sw_k = 0, sw_th = 16, sw_grid = 0, sw_total = 0;
-    size_t sw_lds = 0;
+    // plane sweep over all views: one persistent launch per tile height + one post-processing launch
+    int sw_n = 0, sw_k = 0, sw_ngroups = 0;
+    SweepGroup sw_groups[2];
     int64_t sw_px = 0;
     bool sw_merge = false, sw_final = false;
     void *sw_views = nullptr;     // SweepK[sw_n]
-    int *sw_item0 = nullptr;      // first work item of each view
+    int *sw_vidx = nullptr;       // views ordered by group
+    int *sw_item0 = nullptr;      // first work item of each view within its group (order of sw_vidx)
     int64_t *sw_px0 = nullptr;    // first pixel of each view (sw_n + 1)
-    int *sw_counter = nullptr;    // work queue head
+    int *sw_counter = nullptr;    // work queue heads, one per group
 
-    // PatchMatch over all views: 4 launches per colour pass
-    int pm_n = 0, pm_sm = 0, pm_iters = 0, pm_blk = 0, pm_rows = 0;
+    // PatchMatch over all views: per colour pass, propagation and random refinement once per
+    // source bucket, row ranking and stream scan once for all views
+    int pm_n = 0, pm_iters = 0, pm_rows = 0, pm_ngroups = 0;
+    PMGroup pm_groups[4];
     bool pm_normals = false, pm_prepare = false;
     int64_t pm_px = 0;
-    void *pm_views = nullptr;     // PMK[pm_n]
-    int *pm_blk0 = nullptr, *pm_row0 = nullptr;
-    int64_t *pm_px0 = nullptr;
-    unsigned long long *pm_changed = nullptr;  // (pm_n,)
+    void *pm_views = nullptr;     // PMK[pm_n], ordered by source bucket
+    int *pm_blk0 = nullptr;       // per group: first block of each of its views (nv + 1 each)
+    int *pm_row0 = nullptr;       // first row of each view (pm_n + 1)
+    int64_t *pm_px0 = nullptr;    // first pixel of each view (pm_n + 1)
+    unsigned long long *pm_changed = nullptr;  // (pm_n,) in the caller's view order
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

@@ -58,6 +58,24 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks)
 bool prof_on();
 void prof_push(const char *name, hipEvent_t a, hipEvent_t b);
 
+// Device tables that no kernel writes (view descriptors, camera tables) are read through the
+// constant address space, so that loads at a wave-uniform address become scalar loads instead of
+// 64 lanes fetching the same words.
+#define OM_CONST_AS __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ const OM_CONST_AS T *as_const(const T *p)
+{
+    return (const OM_CONST_AS T *)p;
+}
+
+// a reference to *p whose loads the compiler emits as constant-address-space (scalar) loads
+template <class T>
+__device__ __forceinline__ const T &uniform_ref(const T *p)
+{
+    return *(const T *)as_const(p);
+}
+
 struct ProfScope {
     const char *name = nullptr;
     hipStream_t st = nullptr;

@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <algorithm>
+
 #include "om_common.h"
 
 namespace om {
@@ -122,7 +124,7 @@ struct PMCamArgs {
 __device__ __forceinline__ bool pm_proj(const PMK &P, int s, double X, double Y, double Z, double &u, double &v,
                                         double &z)
 {
-    const double *p = P.cam[s].par;
+    const OM_CONST_AS double *p = as_const(P.cam)[s].par;
     const double dx = X - p[9], dy = Y - p[10], dz = Z - p[11];
     const double xc = p[0] * dx + p[1] * dy + p[2] * dz;
     const double yc = p[3] * dx + p[4] * dy + p[5] * dz;
@@ -220,7 +222,7 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
                     if (s < S) {
                         double pu, pv, pz;
                         if (pm_proj(P, s, X, Y, Z, pu, pv, pz)) {
-                            const PMCam &cs = P.cam[s];
+                            const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
                             const double J = pm_bil(cs.img, cs.iw, cs.ih, pu, pv);
                             if (isfinite(J)) {
                                 sI[u] += I;
@@ -292,7 +294,7 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
         int ng = 0;
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
-            const PMCam &cs = P.cam[s < S ? s : 0];
+            const OM_CONST_AS PMCam &cs = as_const(P.cam)[s < S ? s : 0];
             if (s < S && cs.dep) {
                 ng += 1;
                 double e = 1.0, u, v, z;
@@ -535,7 +537,7 @@ __device__ __forceinline__ int view_of(const int *blk0, int nviews, int b)
     int lo = 0, hi = nviews - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (blk0[mid] <= b)
+        if (as_const(blk0)[mid] <= b)
             lo = mid;
         else
             hi = mid - 1;
@@ -549,7 +551,7 @@ __global__ void __launch_bounds__(kPMThreads) pm_propagate_batch(const PMK *view
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int v = view_of(blk0, nviews, b);
-    pm_propagate_px<SM>(views[v], Q, (int64_t)(b - blk0[v]) * blockDim.x + threadIdx.x);
+    pm_propagate_px<SM>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
 }
 
 template <int SM>
@@ -558,19 +560,19 @@ __global__ void __launch_bounds__(kPMThreads) pm_random_batch(const PMK *views, 
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int v = view_of(blk0, nviews, b);
-    pm_random_px<SM>(views[v], Q, (int64_t)(b - blk0[v]) * blockDim.x + threadIdx.x);
+    pm_random_px<SM>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(256) pm_rank_rows_batch(const PMK *views, const int *row0, int nviews,
                                                           const PassK Q)
 {
     const int v = view_of(row0, nviews, blockIdx.x);
-    pm_rank_row(views[v], Q, blockIdx.x - row0[v]);
+    pm_rank_row(uniform_ref(views + v), Q, blockIdx.x - as_const(row0)[v]);
 }
 
 __global__ void __launch_bounds__(256) pm_scan_rows_batch(const PMK *views, const PassK Q)
 {
-    pm_scan_view(views[blockIdx.x], Q);
+    pm_scan_view(uniform_ref(views + blockIdx.x), Q);
 }
 
 __global__ void pm_init_batch(const PMK *views, int nviews)
@@ -859,19 +861,15 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
     if (step <= 0 || top_k < 1 || iters < 0)
         return fail(OM_EINVAL, "pm_refine needs step >= 1, top_k >= 1, iters >= 0");
     std::vector<PMK> P(nviews);
-    std::vector<int> blk0(nviews + 1, 0), row0(nviews + 1, 0);
-    std::vector<int64_t> px0(nviews + 1, 0);
     b->pm_changed = (unsigned long long *)b->dev_alloc(sizeof(unsigned long long) * nviews);
     if (!b->pm_changed)
         return fail(OM_ENOMEM, "batch: counters");
-    int smax = 1;
     for (int v = 0; v < nviews; ++v) {
         const om_pm_view &V = views[v];
         if (V.S < 1 || V.S > 16)
             return fail(OM_ENSRC, "view %d: PatchMatch needs 1..16 sources", v);
         if (V.H < 1 || V.W < 1)
             return fail(OM_ESHAPE, "view %d: empty image", v);
-        smax = std::max(smax, (int)V.S);
         char *ws = (char *)b->dev_alloc(pm_ws_layout(V.H, V.W, nullptr, nullptr));
         if (!ws)
             return fail(OM_ENOMEM, "batch: PatchMatch workspace");
@@ -927,20 +925,48 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         Q.cam = w.cam;
         Q.depth_from = V.depth_from;
         Q.score_from = V.score_from;
-        const int64_t ncol = (int64_t)V.H * ((V.W + 1) / 2);
-        blk0[v + 1] = blk0[v] + (int)cdiv(ncol, kPMThreads);
-        row0[v + 1] = row0[v] + V.H;
-        px0[v + 1] = px0[v] + (int64_t)V.H * V.W;
+    }
+    // views ordered by source bucket: the propagation / random kernels run once per bucket with
+    // the register footprint of that bucket (4, 8, 12 or 16 sources)
+    std::vector<int> order(nviews);
+    for (int v = 0; v < nviews; ++v)
+        order[v] = v;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return sm_bucket(P[a].S) < sm_bucket(P[c].S); });
+    std::vector<PMK> Ps(nviews);
+    std::vector<int> row0(nviews + 1, 0), gblk0;
+    std::vector<int64_t> px0(nviews + 1, 0);
+    b->pm_ngroups = 0;
+    for (int i = 0; i < nviews; ++i) {
+        const PMK &Q = P[order[i]];
+        Ps[i] = Q;
+        row0[i + 1] = row0[i] + Q.H;
+        px0[i + 1] = px0[i] + (int64_t)Q.H * Q.W;
+        const int sm = sm_bucket(Q.S);
+        if (b->pm_ngroups == 0 || b->pm_groups[b->pm_ngroups - 1].sm != sm) {
+            PMGroup &g = b->pm_groups[b->pm_ngroups++];
+            g.sm = sm;
+            g.first = i;
+            g.nv = 0;
+            g.blk = 0;
+            g.blk0_off = (int)gblk0.size();
+            gblk0.push_back(0);
+        }
+        PMGroup &g = b->pm_groups[b->pm_ngroups - 1];
+        const int64_t ncol = (int64_t)Q.H * ((Q.W + 1) / 2);
+        g.blk += (int)cdiv(ncol, kPMThreads);
+        g.nv += 1;
+        gblk0.push_back(g.blk);
     }
     b->pm_views = b->dev_alloc(sizeof(PMK) * nviews);
-    b->pm_blk0 = (int *)b->dev_alloc(sizeof(int) * (nviews + 1));
+    b->pm_blk0 = (int *)b->dev_alloc(sizeof(int) * gblk0.size());
     b->pm_row0 = (int *)b->dev_alloc(sizeof(int) * (nviews + 1));
     b->pm_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
     if (!b->pm_views || !b->pm_blk0 || !b->pm_row0 || !b->pm_px0)
         return fail(OM_ENOMEM, "batch: PatchMatch tables");
     int rc;
-    if ((rc = hip_check(hipMemcpy(b->pm_views, P.data(), sizeof(PMK) * nviews, hipMemcpyHostToDevice), "upload")) ||
-        (rc = hip_check(hipMemcpy(b->pm_blk0, blk0.data(), sizeof(int) * (nviews + 1), hipMemcpyHostToDevice),
+    if ((rc = hip_check(hipMemcpy(b->pm_views, Ps.data(), sizeof(PMK) * nviews, hipMemcpyHostToDevice), "upload")) ||
+        (rc = hip_check(hipMemcpy(b->pm_blk0, gblk0.data(), sizeof(int) * gblk0.size(), hipMemcpyHostToDevice),
                         "upload")) ||
         (rc = hip_check(hipMemcpy(b->pm_row0, row0.data(), sizeof(int) * (nviews + 1), hipMemcpyHostToDevice),
                         "upload")) ||
@@ -948,9 +974,7 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
                         "upload")))
         return rc;
     b->pm_n = nviews;
-    b->pm_sm = sm_bucket(smax);
     b->pm_iters = iters;
-    b->pm_blk = blk0[nviews];
     b->pm_rows = row0[nviews];
     b->pm_px = px0[nviews];
     b->pm_normals = init_normals != 0;
@@ -960,7 +984,27 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
     return OM_OK;
 }
 
-template <int SM>
+static void pm_group_launch(const om_batch *b, const PMGroup &g, bool propagate, const PassK &Q, hipStream_t st)
+{
+    const PMK *V = (const PMK *)b->pm_views + g.first;
+    const int *blk0 = b->pm_blk0 + g.blk0_off;
+#define OM_PM_GROUP_CASE(SM_)                                                                                          \
+    case SM_:                                                                                                          \
+        if (propagate)                                                                                                 \
+            hipLaunchKernelGGL(pm_propagate_batch<SM_>, dim3((unsigned)g.blk), dim3(kPMThreads), 0, st, V, blk0, g.nv, \
+                               Q);                                                                                     \
+        else                                                                                                           \
+            hipLaunchKernelGGL(pm_random_batch<SM_>, dim3((unsigned)g.blk), dim3(kPMThreads), 0, st, V, blk0, g.nv, Q); \
+        break;
+    switch (g.sm) {
+        OM_PM_GROUP_CASE(4)
+        OM_PM_GROUP_CASE(8)
+        OM_PM_GROUP_CASE(12)
+        OM_PM_GROUP_CASE(16)
+    }
+#undef OM_PM_GROUP_CASE
+}
+
 static void pm_batch_passes(om_batch *b, hipStream_t st)
 {
     const PMK *V = (const PMK *)b->pm_views;
@@ -972,13 +1016,19 @@ static void pm_batch_passes(om_batch *b, hipStream_t st)
             Q.parity = pass & 1;
             Q.dsc = 0.05 / (it + 1);
             Q.nsc = 0.3 / (it + 1);
-            hipLaunchKernelGGL(pm_propagate_batch<SM>, dim3((unsigned)b->pm_blk), dim3(kPMThreads), 0, st, V,
-                               b->pm_blk0, b->pm_n, Q);
+            {
+                ProfScope ps("pm_propagate_batch", st);
+                for (int gi = 0; gi < b->pm_ngroups; ++gi)
+                    pm_group_launch(b, b->pm_groups[gi], true, Q, st);
+            }
             hipLaunchKernelGGL(pm_rank_rows_batch, dim3((unsigned)b->pm_rows), dim3(256), 0, st, V, b->pm_row0,
                                b->pm_n, Q);
             hipLaunchKernelGGL(pm_scan_rows_batch, dim3((unsigned)b->pm_n), dim3(256), 0, st, V, Q);
-            hipLaunchKernelGGL(pm_random_batch<SM>, dim3((unsigned)b->pm_blk), dim3(kPMThreads), 0, st, V,
-                               b->pm_blk0, b->pm_n, Q);
+            {
+                ProfScope ps("pm_random_batch", st);
+                for (int gi = 0; gi < b->pm_ngroups; ++gi)
+                    pm_group_launch(b, b->pm_groups[gi], false, Q, st);
+            }
         }
     }
 }
@@ -993,12 +1043,7 @@ int batch_pm_enqueue(om_batch *b, hipStream_t st)
     if (b->pm_prepare)
         hipLaunchKernelGGL(pm_prepare_batch, dim3((unsigned)cdiv(b->pm_px, 256)), dim3(256), 0, st, V, b->pm_px0,
                            b->pm_n, (int)b->pm_normals);
-    switch (b->pm_sm) {
-    case 4: pm_batch_passes<4>(b, st); break;
-    case 8: pm_batch_passes<8>(b, st); break;
-    case 12: pm_batch_passes<12>(b, st); break;
-    default: pm_batch_passes<16>(b, st); break;
-    }
+    pm_batch_passes(b, st);
     return launch_check("pm batch kernels");
 }
 

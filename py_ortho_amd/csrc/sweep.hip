@@ -72,7 +72,8 @@ struct SweepK {
     SrcK src[16];
 };
 
-__device__ __forceinline__ float hyp_value(const SweepK &P, int i, int p, float a, float b)
+template <class D>  // SweepK, or SweepK in the constant address space (batched launches)
+__device__ __forceinline__ float hyp_value(const D &P, int i, int p, float a, float b)
 {
     // a, b: the generator's per-pixel inputs at p (base/step or iu/ds)
     switch (P.mode) {
@@ -108,8 +109,8 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
 
 // One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
 // device memory (batched launches).
-template <int K, int VR>
-__device__ __forceinline__ void sweep_tile(const SweepK &P, int tile, int chunk, double *lds_d)
+template <int K, int VR, class D>
+__device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, double *lds_d)
 {
     const int r = P.r;
     constexpr int TH = kVSeg * VR;
@@ -194,8 +195,8 @@ __device__ __forceinline__ void sweep_tile(const SweepK &P, int tile, int chunk,
         __syncthreads();
 
         for (int s = 0; s < P.n; ++s) {
-            const SrcK &S = P.src[s];
-            const double *g = S.g;
+            const auto &S = P.src[s];
+            const auto *g = S.g;
             const double wl = (double)S.w - 1.001, hl = (double)S.h - 1.001;
             const float *img = S.img;
             const int sw = S.w;
@@ -375,12 +376,12 @@ __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
     sweep_tile<K, VR>(P, lb / P.chunks, lb % P.chunks, lds_d);
 }
 
-// Persistent batch: work items (view, tile, chunk) of many views from one queue.  Every wave of a
-// workgroup reads the same item index from LDS, so all of them leave the loop together once the
-// queue is empty.
+// Persistent batch: work items (view, tile, chunk) of the views vidx[0..nv) from one queue; item0[j]
+// is the first item of view vidx[j].  Every wave of a workgroup reads the same item index from LDS,
+// so all of them leave the loop together once the queue is empty.
 template <int K, int VR>
-__global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *views, const int *item0, int nviews,
-                                                               int total, int *counter)
+__global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *views, const int *vidx, const int *item0,
+                                                               int nv, int total, int *counter)
 {
     extern __shared__ double lds_d[];
     __shared__ int s_item;
@@ -389,19 +390,19 @@ __global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *vie
         if (threadIdx.x == 0)
             s_item = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = s_item;
+        const int it = __builtin_amdgcn_readfirstlane(s_item);  // the same for every lane
         if (it >= total)
             break;
-        int lo = 0, hi = nviews - 1;  // last view whose first item is <= it
+        int lo = 0, hi = nv - 1;  // last view whose first item is <= it
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (item0[mid] <= it)
+            if (as_const(item0)[mid] <= it)
                 lo = mid;
             else
                 hi = mid - 1;
         }
-        const SweepK &P = views[lo];
-        const int local = it - item0[lo];
+        const OM_CONST_AS SweepK &P = as_const(views)[as_const(vidx)[lo]];
+        const int local = it - as_const(item0)[lo];
         sweep_tile<K, VR>(P, local / P.chunks, local % P.chunks, lds_d);
     }
 }
@@ -446,7 +447,8 @@ __global__ void sweep_merge_kernel(const float *__restrict__ part, int chunks, i
 }
 
 // parabola of densify._sweep (densify.py:354-361), NumPy's float32 / float64 promotion kept
-__device__ __forceinline__ void finalize_px(const SweepK &P, int64_t p, float *depth)
+template <class D>
+__device__ __forceinline__ void finalize_px(const D &P, int64_t p, float *depth)
 {
     const float a = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.ha[p] : 0.f;
     const float b = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.hb[p] : 0.f;
@@ -502,7 +504,11 @@ __global__ void sweep_batch_post_kernel(const SweepK *views, const int64_t *px0,
         finalize_px(P, p, P.depth_out);
 }
 
-__global__ void reset_counter_kernel(int *c) { *c = 0; }
+__global__ void reset_counters_kernel(int *c, int n)
+{
+    if ((int)threadIdx.x < n)
+        c[threadIdx.x] = 0;
+}
 
 // a batch run starts every view from the initial winner state of densify._sweep
 // (densify.py:331-335): best = prev = s_prev_best = s_next_best = -2, idx = -1
@@ -704,6 +710,8 @@ extern "C" int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *str
         return rc;
     if (d->i0 != 0)
         return fail(OM_EINVAL, "om_sweep_finalize expects the full hypothesis range (i0 = 0)");
+    if (d->nh < 1)
+        return fail(OM_EINVAL, "the parabola needs at least one hypothesis");
     if (d->hyp_mode == OM_HYP_MAPS && d->nh > 1 && !d->hyp_a)
         return fail(OM_EINVAL, "null hypothesis maps");
     const int64_t HW = (int64_t)P.H * P.W;
@@ -721,7 +729,7 @@ extern "C" int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *str
 
 namespace om {
 
-using BatchFn = void (*)(const SweepK *, const int *, int, int, int *);
+using BatchFn = void (*)(const SweepK *, const int *, const int *, int, int, int *);
 
 static BatchFn batch_kernel(int k, int th)
 {
@@ -742,12 +750,15 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
 {
     if (nviews < 1 || !views)
         return fail(OM_EINVAL, "batch needs at least one view");
+    if (tile_h != 0 && tile_h != 8 && tile_h != 16)
+        return fail(OM_EINVAL, "tile height must be 8 or 16 rows (got %d)", tile_h);
     std::vector<SweepK> P(nviews);
     int k = 0, r = -1;
     int64_t tiles = 0;
     for (int v = 0; v < nviews; ++v) {
         om_sweep_desc d = views[v];
-        d.tile_h = tile_h;
+        if (tile_h)
+            d.tile_h = tile_h;
         int rc = build_params(&d, P[v]);
         if (rc)
             return fail(rc, "view %d: %s", v, om_last_error());
@@ -759,21 +770,22 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
         }
         if (d.i0 != 0)  // every run starts each view from the initial winner state
             return fail(OM_EINVAL, "view %d: a batch sweeps hypotheses from i0 = 0", v);
+        if (depth_out && depth_out[v] && d.nh < 1)
+            return fail(OM_EINVAL, "view %d: the parabola needs at least one hypothesis", v);
         tiles += (int64_t)P[v].tiles_x * P[v].tiles_y;
     }
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    int c = chunks;
-    if (c <= 0)
-        c = (int)std::min<int64_t>(std::max<int64_t>(cdiv(4LL * cus, std::max<int64_t>(tiles, 1)), 1), 64);
-    std::vector<int> item0(nviews);
+    // chunks: the batch argument for all views, else a view's own desc.chunks, else enough chunks
+    // for about 4 workgroups per CU over the whole batch
+    const int c_auto = (int)std::min<int64_t>(std::max<int64_t>(cdiv(4LL * cus, std::max<int64_t>(tiles, 1)), 1), 64);
     std::vector<int64_t> px0(nviews + 1, 0);
-    int64_t total = 0;
     bool merge = false, final_ = false;
     for (int v = 0; v < nviews; ++v) {
         SweepK &Q = P[v];
+        const int c = chunks > 0 ? chunks : (views[v].chunks > 0 ? views[v].chunks : c_auto);
         const int cv = std::max(1, std::min(c, std::max(Q.nh, 1)));
         Q.hc = (int)cdiv(std::max(Q.nh, 1), cv);
         Q.chunks = (int)cdiv(std::max(Q.nh, 1), Q.hc);
@@ -786,39 +798,57 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
         }
         Q.depth_out = depth_out ? depth_out[v] : nullptr;
         final_ = final_ || Q.depth_out;
-        item0[v] = (int)total;
-        total += (int64_t)Q.tiles_x * Q.tiles_y * Q.chunks;
         px0[v + 1] = px0[v] + HW;
     }
-    if (total >= INT_MAX)
-        return fail(OM_EUNSUP, "batch has too many work items");
+    // one persistent launch per tile height, each with its own queue
+    std::vector<int> vidx, item0;
+    b->sw_ngroups = 0;
+    for (int th : {16, 8}) {
+        SweepGroup &g = b->sw_groups[b->sw_ngroups];
+        g.th = th;
+        g.first = (int)vidx.size();
+        int64_t total = 0;
+        for (int v = 0; v < nviews; ++v) {
+            if (P[v].TH != th)
+                continue;
+            vidx.push_back(v);
+            item0.push_back((int)total);
+            total += (int64_t)P[v].tiles_x * P[v].tiles_y * P[v].chunks;
+            if (total >= INT_MAX)
+                return fail(OM_EUNSUP, "batch has too many work items");
+        }
+        g.nv = (int)vidx.size() - g.first;
+        if (g.nv == 0 || total == 0)
+            continue;
+        g.total = (int)total;
+        g.lds = sweep_lds_bytes(th, r);
+        int per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th), kThreads, g.lds) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        g.grid = grid_blocks > 0 ? grid_blocks : (int)std::min<int64_t>((int64_t)per_cu * cus, total);
+        b->sw_ngroups++;
+    }
     b->sw_views = b->dev_alloc(sizeof(SweepK) * nviews);
+    b->sw_vidx = (int *)b->dev_alloc(sizeof(int) * nviews);
     b->sw_item0 = (int *)b->dev_alloc(sizeof(int) * nviews);
     b->sw_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
-    b->sw_counter = (int *)b->dev_alloc(sizeof(int));
-    if (!b->sw_views || !b->sw_item0 || !b->sw_px0 || !b->sw_counter)
+    b->sw_counter = (int *)b->dev_alloc(sizeof(int) * 2);
+    if (!b->sw_views || !b->sw_vidx || !b->sw_item0 || !b->sw_px0 || !b->sw_counter)
         return fail(OM_ENOMEM, "batch: descriptor tables");
     int rc;
     if ((rc = hip_check(hipMemcpy(b->sw_views, P.data(), sizeof(SweepK) * nviews, hipMemcpyHostToDevice), "upload")) ||
+        (rc = hip_check(hipMemcpy(b->sw_vidx, vidx.data(), sizeof(int) * nviews, hipMemcpyHostToDevice), "upload")) ||
         (rc = hip_check(hipMemcpy(b->sw_item0, item0.data(), sizeof(int) * nviews, hipMemcpyHostToDevice), "upload")) ||
         (rc = hip_check(hipMemcpy(b->sw_px0, px0.data(), sizeof(int64_t) * (nviews + 1), hipMemcpyHostToDevice),
                         "upload")))
         return rc;
     b->sw_n = nviews;
     b->sw_k = k;
-    b->sw_th = P[0].TH;
-    b->sw_total = (int)total;
     b->sw_px = px0[nviews];
     b->sw_merge = merge;
     b->sw_final = final_;
-    b->sw_lds = sweep_lds_bytes(b->sw_th, r);
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, b->sw_th), kThreads,
-                                                     b->sw_lds) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    const int resident = per_cu * cus;
-    b->sw_grid = grid_blocks > 0 ? grid_blocks : (int)std::min<int64_t>(resident, total);
     return OM_OK;
 }
 
@@ -827,11 +857,16 @@ int batch_sweep_enqueue(om_batch *b, hipStream_t st)
     if (b->sw_n == 0)
         return OM_OK;
     ProfScope ps("sweep_batch", st);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, b->sw_counter);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, b->sw_counter, 2);
     hipLaunchKernelGGL(sweep_batch_reset_kernel, dim3((unsigned)cdiv(b->sw_px, 256)), dim3(256), 0, st,
                        (const SweepK *)b->sw_views, b->sw_px0, b->sw_n);
-    hipLaunchKernelGGL(batch_kernel(b->sw_k, b->sw_th), dim3((unsigned)b->sw_grid), dim3(kThreads), b->sw_lds, st,
-                       (const SweepK *)b->sw_views, b->sw_item0, b->sw_n, b->sw_total, b->sw_counter);
+    for (int gi = 0; gi < b->sw_ngroups; ++gi) {
+        const SweepGroup &g = b->sw_groups[gi];
+        ProfScope pk("sweep_batch_kernel", st);
+        hipLaunchKernelGGL(batch_kernel(b->sw_k, g.th), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st,
+                           (const SweepK *)b->sw_views, b->sw_vidx + g.first, b->sw_item0 + g.first, g.nv, g.total,
+                           b->sw_counter + gi);
+    }
     int rc = launch_check("sweep_batch_kernel");
     if (rc)
         return rc;

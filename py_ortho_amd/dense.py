@@ -60,6 +60,11 @@ class SweepState:
     def tensors(self):
         return self.best, self.prev, self.s_prev_best, self.s_next_best, self.idx
 
+    def reset(self):
+        for t in (self.best, self.prev, self.s_prev_best, self.s_next_best):
+            t.fill_(-2.0)
+        self.idx.fill_(-1)
+
 
 def _src_array(srcs: Sequence[torch.Tensor], geo: np.ndarray):
     n = len(srcs)

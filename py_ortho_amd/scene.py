@@ -243,3 +243,78 @@ def rays_world(cam: Camera, device="cpu") -> torch.Tensor:
     """(3, h, w) float32 rays of the reference camera as densify._rays_world builds them for a
     pinhole camera (densify.py:288-297): x = (u - cx) / f, y = (v - cy) / f, then R^T."""
     return camera_rays(cam, device).to(torch.float32).contiguous()
+
+
+# ---- multi-view survey blocks -------------------------------------------------------------------
+
+C3_SIZES = ((800, 600), (1200, 900), (1600, 1200), (2400, 1800))
+
+
+@dataclass
+class Survey:
+    """A block of nadir views over one terrain; nbrs[k] are the source views of view k."""
+    terrain: Terrain
+    views: List[View]
+    nbrs: List[List[int]]
+    dmin: float
+    dmax: float
+
+    def geo(self, k: int) -> np.ndarray:
+        return np.stack([self.views[j].cam.geo18(self.views[k].cam) for j in self.nbrs[k]])
+
+    def par(self, k: int) -> np.ndarray:
+        return np.stack([self.views[j].cam.par18() for j in self.nbrs[k]])
+
+
+def neighbours_by_distance(C: np.ndarray, alt: float, counts: Sequence[int], min_angle_deg: float = 8.0):
+    """For each camera the counts[k] nearest others (ties by index) whose baseline subtends at least
+    min_angle_deg at the flying height -- the >= 8 degree triangulation preference of
+    densify._neighbours (densify.py:198-201) without tie-point counts."""
+    out = []
+    for k in range(len(C)):
+        d = np.linalg.norm(C - C[k], axis=1)
+        ang = np.degrees(d / alt)
+        cand = [j for j in np.lexsort((np.arange(len(C)), d)) if j != k and ang[j] >= min_angle_deg]
+        out.append([int(j) for j in cand[:counts[k]]])
+    return out
+
+
+def survey_cameras(seed: int, nx: int, ny: int, alt: float, sizes: Sequence[Tuple[int, int]], f_per_w: float,
+                   forward: float, side: float, s_range: Tuple[int, int]):
+    """nx x ny nadir grid with the given forward (x) / side (y) overlaps of the image footprint; each
+    view's size drawn uniformly from `sizes`, f = f_per_w * W (same footprint for every size), and
+    its neighbour count from U{s_range}.  -> (cameras, counts, terrain extent)."""
+    rng = np.random.default_rng(seed)
+    W0, H0 = sizes[0]
+    foot_w, foot_h = alt / f_per_w, alt / f_per_w * H0 / W0
+    dx, dy = (1 - forward) * foot_w, (1 - side) * foot_h
+    cams, counts = [], []
+    for iy in range(ny):
+        for ix in range(nx):
+            W, H = sizes[int(rng.integers(len(sizes)))]
+            x, y = (ix - (nx - 1) / 2) * dx, (iy - (ny - 1) / 2) * dy
+            cams.append(pinhole((x, y, alt), f_per_w * W, W, H))
+            counts.append(int(rng.integers(s_range[0], s_range[1] + 1)))
+    ext = (-(nx - 1) / 2 * dx - foot_w, (nx - 1) / 2 * dx + foot_w, -(ny - 1) / 2 * dy - foot_h,
+           (ny - 1) / 2 * dy + foot_h)
+    return cams, counts, ext
+
+
+def make_survey(seed: int, cams: Sequence[Camera], counts: Sequence[int], ext, alt: float, device="cpu",
+                noise: float = 0.01, hmax: float = 15.0, n_bumps: int = 40) -> Survey:
+    terr = make_terrain(seed, ext, n_bumps=n_bumps, hmax=hmax)
+    views = []
+    for k, cam in enumerate(cams):
+        img, dep = render(terr, cam, noise_seed=seed * 1000 + 2 + k, noise=noise, device=device)
+        views.append(View(cam, img, dep))
+    C = np.stack([c.C for c in cams])
+    nbrs = neighbours_by_distance(C, alt, counts)
+    return Survey(terr, views, nbrs, dmin=alt - hmax - 2.0, dmax=alt + 2.0)
+
+
+def config_c3(device="cpu", scale: float = 1.0) -> Survey:
+    """Batched block: 8 x 6 nadir grid at 90 m with 70 % forward / 60 % side overlap, 48 views whose
+    sizes are drawn from 800x600 .. 2400x1800 (f = 0.9375 W), S = U{4..12} nearest views (seed 4)."""
+    sizes = [(int(round(w * scale)), int(round(h * scale))) for w, h in C3_SIZES]
+    cams, counts, ext = survey_cameras(4, 8, 6, 90.0, sizes, 0.9375, 0.7, 0.6, (4, 12))
+    return make_survey(4, cams, counts, ext, 90.0, device=device)

@@ -52,19 +52,26 @@ def _sweep_views(seed=0, r=3, k=2):
     return out
 
 
-@pytest.mark.parametrize("chunks,tile_h", [(0, 0), (1, 16), (3, 8), (2, 16)])
+@pytest.mark.parametrize("chunks,tile_h", [(0, 0), (1, 16), (3, 8), (2, 16), ("mixed", "mixed")])
 def test_batch_sweep_equals_oracle_per_view(chunks, tile_h):
+    """Every view equals the oracle's sweep + parabola; "mixed" gives each view its own tile height
+    and chunk count (two persistent launches, one per tile height)."""
     from py_ortho_amd import batch
     r, k = 3, 2
     vs = _sweep_views(1, r, k)
     B = batch.Batch()
     bviews = []
-    for v in vs:
+    for j, v in enumerate(vs):
         H, W = v["ref"].shape
         bviews.append(batch.SweepView(_dev(v["ref"]), _dev(v["mu"]), _dev(v["sd"]), [_dev(s) for s in v["srcs"]],
                                       _dev(v["ray"]), v["geo"], v["hyp"],
-                                      depth=torch.empty((H, W), dtype=torch.float32, device="cuda")))
-    B.set_sweep(bviews, r, k, chunks=chunks, tile_h=tile_h)
+                                      depth=torch.empty((H, W), dtype=torch.float32, device="cuda"),
+                                      tile_h=(8, 16)[j % 2] if chunks == "mixed" else 0,
+                                      chunks=(1, 2, 3)[j % 3] if chunks == "mixed" else 0))
+    if chunks == "mixed":
+        B.set_sweep(bviews, r, k)
+    else:
+        B.set_sweep(bviews, r, k, chunks=chunks, tile_h=tile_h)
     B.run()
     torch.cuda.synchronize()
     for v, bv in zip(vs, bviews):
