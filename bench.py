@@ -59,6 +59,17 @@ def _percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else None
 
 
+def profiled_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel from the committed counter summary (two
+    rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, calibrated against a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    import re
+    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def prof_read(lib, name):
     tot, n = ctypes.c_double(0), ctypes.c_int64(0)
     rc = lib.om_profile_read(name.encode(), ctypes.byref(tot), ctypes.byref(n))
@@ -162,8 +173,8 @@ def run_c2(args, torch):
                    "H": H, "W": W, "sources": S, "iterations": c["iters"], "pixel_hypotheses_per_step": evals},
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
         "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)", "achieved": achieved / 1e12,
-                     "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None,
-                     "kernel": "pm_propagate_kernel", "kernel_avg_ms": prop_avg * 1e3, "launches": prop_n,
+                     "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
+                     "traffic": profiled_traffic("pm_propagate_c2.md"), "kernel": "pm_propagate_kernel", "kernel_avg_ms": prop_avg * 1e3, "launches": prop_n,
                      "flops_per_launch": prop_flops, "flops_per_eval": fpe,
                      "pm_random_avg_ms": rand_ms / max(rand_n, 1)},
         "accuracy": acc,
