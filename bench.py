@@ -485,12 +485,209 @@ def block_geo(blk, j, k):
     return _geo18(blk.cams[j], blk.cams[k])
 
 
+# ------------------------------------------------------------------------------- C5 served depth maps
+def _pools_c5(torch, cache, scale):
+    """The C5 image pools, rendered on the device or loaded from `cache` (written on first use)."""
+    from py_ortho_amd import scene
+    if cache and os.path.exists(cache):
+        z = np.load(cache, allow_pickle=False)
+        pools = []
+        for c, (w, h, _) in enumerate(scene.C5_CLASSES):
+            size = (int(round(w * scale)), int(round(h * scale)))
+            cams, counts, ext = scene.survey_cameras(6 + c, 4, 4, 90.0, [size], 0.9375, 0.7, 0.6, (12, 12))
+            views = [scene.View(cm, torch.from_numpy(z[f"img{c}_{k}"]).cuda(), torch.from_numpy(z[f"dep{c}_{k}"]).cuda())
+                     for k, cm in enumerate(cams)]
+            C = np.stack([cm.C for cm in cams])
+            pools.append(scene.Survey(None, views, scene.neighbours_by_distance(C, 90.0, counts), 73.0, 92.0))
+        return pools
+    pools = scene.config_c5_pools(device="cuda", scale=scale)
+    if cache:
+        np.savez(cache, **{f"img{c}_{k}": v.img.cpu().numpy() for c, sv in enumerate(pools) for k, v in enumerate(sv.views)},
+                 **{f"dep{c}_{k}": v.depth.cpu().numpy() for c, sv in enumerate(pools) for k, v in enumerate(sv.views)})
+    return pools
+
+
+def c5_requests(pools, rate, duration, seed=6):
+    """Poisson arrivals at `rate`/s for `duration` s: size class 60/30/10 %, S = U{4..12} nearest
+    views, planes U{128..384} -> [(arrival offset s, DepthRequest)]."""
+    from py_ortho_amd import scene
+    from py_ortho_amd.serve import DepthRequest
+    rng = np.random.default_rng(seed)
+    probs = np.array([p for _, _, p in scene.C5_CLASSES])
+    out, t, i = [], 0.0, 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        c = int(rng.choice(len(pools), p=probs / probs.sum()))
+        sv = pools[c]
+        k = int(rng.integers(len(sv.views)))
+        S = int(rng.integers(4, 13))
+        out.append((t, DepthRequest((c, k), [(c, j) for j in sv.nbrs[k][:S]], sv.dmin, sv.dmax,
+                                    int(rng.integers(128, 385)), tag=i)))
+        i += 1
+    return out
+
+
+def run_c5(args, torch):
+    import threading
+    from py_ortho_amd import block, dense, serve
+    t_scene = time.perf_counter()
+    pools = _pools_c5(torch, args.scene_cache, args.scale)
+    torch.cuda.synchronize()
+    t_scene = time.perf_counter() - t_scene
+    images = {(c, k): (v.img.cpu().numpy(), v.cam) for c, sv in enumerate(pools) for k, v in enumerate(sv.views)}
+    truth = {(c, k): v.depth for c, sv in enumerate(pools) for k, v in enumerate(sv.views)}
+    window, top_k = 5, 2
+    ex = serve.GpuExecutor(window=window, top_k=top_k)
+    store = serve.ImageStore(lambda key: images[key], args.budget_gb * 1e9, upload=ex.upload)
+    srv = serve.DepthServer(store, ex, budget_bytes=args.budget_gb * 1e9, step_planes=args.step_planes).start()
+    # warm-up: two small requests (code objects, allocator)
+    warm = c5_requests(pools, 50.0, 0.05, seed=99)[:2]
+    for _, r in warm:
+        r.planes = 16
+    for f in [srv.submit(r) for _, r in warm]:
+        f.result()
+    arrivals = c5_requests(pools, args.rate, args.duration)
+    futs = [None] * len(arrivals)
+    t0 = time.perf_counter()
+
+    def feed():
+        for j, (ta, req) in enumerate(arrivals):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs[j] = srv.submit(req, t_arrival=t0 + ta)
+
+    from py_ortho_amd import _lib
+    lib = _lib.lib()
+    lib.om_profile_enable(1)
+    prof_read(lib, "sweep_multi")
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    res = [f.result() for f in futs]
+    srv.stop()
+    lib.om_profile_enable(0)
+    step_ms, step_n = prof_read(lib, "sweep_multi")
+    lib.om_profile_read(None, None, None)
+    t_end = max(r.t_done for r in res)
+    px_h = [images[q.ref][0].size * q.planes for _, q in arrivals]
+    flops = sum(images[q.ref][0].size * q.planes * sweep_flops_per_hyp(len(q.srcs)) for _, q in arrivals)
+    wall = t_end - t0
+    lat = [r.latency * 1e3 for r in res]
+    served = sum(px_h) / wall / 1e6
+    # plain per-request runs (one request at a time: fused single-view sweep + parabola), same inputs
+    n_plain = min(len(arrivals), args.plain_requests)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    same = True
+    for j in range(n_plain):
+        q = arrivals[j][1]
+        img, cam = images[q.ref]
+        ref = torch.from_numpy(img).cuda()
+        ref = torch.where(torch.isfinite(ref), ref, torch.zeros_like(ref)).contiguous()
+        H, W = ref.shape
+        mu, sd = dense.ref_stats(ref, window)
+        ray = dense.rays_world(H, W, cam.R, cam.cx, cam.cy, cam.f, cam.k1, cam.k2, cam.k3)
+        srcs = [store.pin(k) for k in q.srcs]
+        geo = np.stack([block._geo18(images[k][1], cam) for k in q.srcs])
+        hyp = {"mode": "const", "inv": torch.from_numpy(block.inverse_depth_planes(q.dmin, q.dmax, q.planes)).cuda()}
+        st = dense.sweep(ref, mu, sd, srcs, ray, geo, hyp, window, top_k)
+        d = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, window, top_k, st).cpu().numpy()
+        for k in q.srcs:
+            store.unpin(k)
+        same = same and np.array_equal(d.view(np.uint32), res[j].depth.view(np.uint32))
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp
+    plain = sum(px_h[:n_plain]) / plain_s / 1e6
+    rel = []
+    for (ta, q), r in zip(arrivals, res):
+        t = truth[q.ref].cpu().numpy()
+        e = np.abs(r.depth - t) / t
+        rel.append(e[np.isfinite(e)])
+    rel = np.concatenate(rel)
+    classes = {}
+    for (ta, q), r in zip(arrivals, res):
+        H, W = images[q.ref][0].shape
+        classes.setdefault(f"{W}x{H}", []).append(r.latency * 1e3)
+    res_line = {
+        "metric": METRIC, "value": served, "unit": "Mpixel-hyp/s", "n_gpus": 1, "steps": len(arrivals),
+        "warmup": len(warm), "ms_per_step": wall * 1e3 / max(len(arrivals), 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"C5 served depth maps: Poisson arrivals {args.rate:g}/s for {args.duration:g} s, sizes "
+                               "1200x900 / 2400x1800 / 4000x3000 (60/30/10 pct)"
+                               f" x {args.scale:g}, S=U{{4..12}}, planes U{{128..384}}, 11x11 NCC top-2 + parabola, "
+                               f"{args.budget_gb:g} GB admission budget, continuous batching ({args.step_planes} "
+                               "planes per step)",
+                   "requests": len(arrivals), "pixel_hypotheses": int(sum(px_h)), "wall_s": wall,
+                   "offered_mpix_hyp_s": sum(px_h) / args.duration / 1e6, "steps": srv.steps,
+                   "max_active": srv.stats["max_active"], "image_uploads": store.uploads, "scene_s": t_scene},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99),
+                       "by_class_p50": {k: _percentile(v, 50) for k, v in classes.items()},
+                       "by_class_p99": {k: _percentile(v, 99) for k, v in classes.items()}},
+        "plain_per_request": {"requests": n_plain, "mpix_hyp_s": plain, "speedup": served / plain,
+                              "served_equals_plain": bool(same)},
+        "accuracy": {"frac_within_1pct": float((rel < 0.01).mean()), "median_rel_err": float(np.median(rel))},
+        "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
+                     "achieved": flops / (step_ms / 1e3) / 1e12 if step_ms else 0.0, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": flops / (step_ms / 1e3) / FP64_PEAK if step_ms else 0.0,
+                     "traffic": None, "kernel": "sweep_batch_kernel (om_sweep_multi steps)",
+                     "kernel_avg_ms": step_ms / max(step_n, 1), "launches": step_n,
+                     "device_busy_frac": step_ms / 1e3 / wall},
+    }
+    if not args.no_cpu:
+        res_line["cpu_baseline"] = cpu_baseline_sweep(images, arrivals, window, top_k, args)
+    return res_line
+
+
+def cpu_baseline_sweep(images, arrivals, window, top_k, args):
+    """The oracle's sweep on a centre crop of the first request with 8 sources (its planes), `cores`
+    concurrent copies: Mpixel-hyp/s = cores * crop pixels * planes / wall."""
+    import oracle
+    from py_ortho_amd import block
+    cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
+    q = next((q for _, q in arrivals if len(q.srcs) == 8), arrivals[0][1])
+    img, cam = images[q.ref]
+    H, W = img.shape
+    ref = np.nan_to_num(img).astype(np.float32)
+    mu, sd = oracle.ref_stats(ref, window)
+    ray = oracle.rays_world(H, W, cam.R, cam.cx, cam.cy, cam.f)
+    srcs = [images[k][0] for k in q.srcs]
+    geo = np.stack([block._geo18(images[k][1], cam) for k in q.srcs])
+    planes = block.inverse_depth_planes(q.dmin, q.dmax, q.planes)
+
+    def sweep(side, nd, threads):
+        y0, x0 = (H - side) // 2, (W - side) // 2
+        sl = np.s_[y0:y0 + side, x0:x0 + side]
+        base = np.full((side, side), planes[0], np.float32)
+        step = np.full((side, side), planes[1] - planes[0], np.float32)
+        t = time.perf_counter()
+        oracle.bench_sweep(threads, ref[sl], mu[sl], sd[sl], srcs, np.ascontiguousarray(ray[:, y0:y0 + side,
+                                                                                         x0:x0 + side]),
+                           geo, base, step, nd, window, top_k)
+        return time.perf_counter() - t
+
+    per = sweep(16, 8, 1) / (256 * 8)
+    side = int(max(16, min(512, math.sqrt(1.5 / (per * q.planes)))))
+    wall = sweep(side, q.planes, cores)
+    return {"value": cores * side * side * q.planes / wall / 1e6, "unit": "Mpixel-hyp/s", "cores": cores,
+            "kind": "port", "sample": f"{cores} concurrent copies of the oracle sweep ({q.planes} planes, S="
+                                      f"{len(q.srcs)}, 11x11 window) on the {side}x{side} centre crop of a {W}x{H} "
+                                      f"request ({wall:.1f} s wall)"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c5"], default="c2")
+    ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
+    ap.add_argument("--duration", type=float, default=3.0, help="C5: seconds of arrivals")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="C5: admission budget (GB of HBM)")
+    ap.add_argument("--step-planes", type=int, default=32, help="C5: planes per request per step")
+    ap.add_argument("--plain-requests", type=int, default=24, help="C5: requests timed one at a time")
     ap.add_argument("--scale", type=float, default=1.0, help="C3: image size scale (1 = the full block)")
     ap.add_argument("--plain-per-hyp", action="store_true",
                     help="C3: also time the per-view form with one sweep launch per plane")
@@ -516,7 +713,7 @@ def main(argv=None):
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3}[args.workload](args, torch)
+    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c5": run_c5}[args.workload](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -180,6 +180,18 @@ int om_batch_replay(om_batch *b, void *stream);
 /* accepted PatchMatch changes per view (synchronises the device) */
 int om_batch_pm_changed(om_batch *b, int64_t *changed);
 
+/* One step of a continuous-batching server: for every view, hypotheses [i0, i0 + nh) continuing its
+ * winner state (views of any size, source count and hypothesis generator; same r and min(top_k, n)),
+ * in one persistent launch per tile height; views with depth_out[v] != NULL finish (their i0 + nh is
+ * the whole sweep) and get the parabola depth.  Equal, view by view, to om_sweep over the same range
+ * followed by om_sweep_finalize.  The call allocates nothing: table is device memory and staging
+ * pinned host memory (hipHostMalloc), both of om_sweep_multi_bytes(nviews); the table is uploaded from
+ * staging on `stream`, so staging may be reused once the stream has passed this call.  Replaces the
+ * per-view thread pool of densify.py:596 for a stream of requests. */
+size_t om_sweep_multi_bytes(int32_t nviews);
+int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float *const *depth_out, int32_t tile_h, void *table,
+                   size_t table_bytes, void *staging, void *stream);
+
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);
 

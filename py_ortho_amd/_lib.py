@@ -102,6 +102,9 @@ SIGNATURES = {
     "om_batch_replay": (c_int32, [c_void_p, c_void_p]),
     "om_batch_pm_changed": (c_int32, [c_void_p, ctypes.POINTER(c_int64)]),
     "om_init_normals": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
+    "om_sweep_multi_bytes": (c_size_t, [c_int32]),
+    "om_sweep_multi": (c_int32, [ctypes.POINTER(OmSweepDesc), c_int32, ctypes.POINTER(c_void_p), c_int32, c_void_p,
+                                 c_size_t, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -69,6 +69,7 @@ struct SweepK {
     int *idx;
     float *part;              // (chunks, 6, H, W) when chunks > 1
     float *depth_out;         // batched launches: parabola depth of the view, or null
+    int fin_n;                // hypotheses of the whole sweep (the parabola's n): i0 + nh
     SrcK src[16];
 };
 
@@ -89,7 +90,7 @@ __device__ __forceinline__ float hyp_value(const D &P, int i, int p, float a, fl
     case OM_HYP_CONST:
         return P.ha[i - P.i0];
     default:
-        return P.ha[(size_t)(i - P.i0) * ((size_t)P.H * P.W) + p];
+        return P.ha[(int64_t)(i - P.i0) * ((int64_t)P.H * P.W) + p];
     }
 }
 
@@ -452,9 +453,9 @@ __device__ __forceinline__ void finalize_px(const D &P, int64_t p, float *depth)
 {
     const float a = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.ha[p] : 0.f;
     const float b = (P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE) ? P.hb[p] : 0.f;
-    const float inv0 = hyp_value(P, P.i0, (int)p, a, b);
-    const int n = P.nh;
-    const float dinv = n > 1 ? hyp_value(P, P.i0 + 1, (int)p, a, b) - inv0 : 0.0f;
+    const float inv0 = hyp_value(P, 0, (int)p, a, b);
+    const int n = P.fin_n;
+    const float dinv = n > 1 ? hyp_value(P, 1, (int)p, a, b) - inv0 : 0.0f;
     const float best = P.best[p], spb = P.spb[p], snb = P.snb[p];
     const int idx = P.idx[p];
     const float den = (spb - 2.0f * best) + snb;
@@ -502,6 +503,17 @@ __global__ void sweep_batch_post_kernel(const SweepK *views, const int64_t *px0,
         merge_px(P.part, P.chunks, HW, P.best, P.prev, P.spb, P.snb, P.idx, P.i0, P.hc, p);
     if (do_final && P.depth_out)
         finalize_px(P, p, P.depth_out);
+}
+
+// parabola of the views fidx[0..nf): view fidx[j] owns pixels [fpx0[j], fpx0[j+1])
+__global__ void sweep_multi_finalize_kernel(const SweepK *views, const int *fidx, const int64_t *fpx0, int nf)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fpx0[nf])
+        return;
+    const int j = view_of_px(fpx0, nf, t);
+    const OM_CONST_AS SweepK &P = as_const(views)[as_const(fidx)[j]];
+    finalize_px(P, t - fpx0[j], P.depth_out);
 }
 
 __global__ void reset_counters_kernel(int *c, int n)
@@ -563,6 +575,7 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
     P.hb = d->hyp_b;
     P.hoff = d->hyp_offset;
     P.i0 = d->i0;
+    P.fin_n = d->i0 + d->nh;
     P.nh = d->nh;
     P.best = d->best;
     P.prev = d->prev;
@@ -879,3 +892,145 @@ int batch_sweep_enqueue(om_batch *b, hipStream_t st)
 }
 
 }  // namespace om
+
+// ------------------------------------------------------------------------------------------
+// one-shot multi-view sweep step (om_sweep_multi): hypotheses [i0, i0 + nh) of every view,
+// continuing each view's winner state -- the step of a continuous-batching server
+// ------------------------------------------------------------------------------------------
+namespace om {
+
+struct MultiLayout {
+    size_t views, vidx, item0, fidx, fpx0, counter, total;
+};
+
+static size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static MultiLayout multi_layout(int n)
+{
+    MultiLayout L;
+    size_t o = 0;
+    L.views = o;
+    o += al256(sizeof(SweepK) * n);
+    L.vidx = o;
+    o += al256(sizeof(int) * n);
+    L.item0 = o;
+    o += al256(sizeof(int) * n);
+    L.fidx = o;
+    o += al256(sizeof(int) * n);
+    L.fpx0 = o;
+    o += al256(sizeof(int64_t) * (n + 1));
+    L.counter = o;
+    o += 256;
+    L.total = o;
+    return L;
+}
+
+}  // namespace om
+
+extern "C" size_t om_sweep_multi_bytes(int32_t nviews)
+{
+    return nviews < 1 ? 0 : multi_layout(nviews).total;
+}
+
+extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float *const *depth_out, int32_t tile_h,
+                              void *table, size_t table_bytes, void *staging, void *stream)
+{
+    if (nviews < 1 || !views)
+        return fail(OM_EINVAL, "om_sweep_multi needs at least one view");
+    if (tile_h != 0 && tile_h != 8 && tile_h != 16)
+        return fail(OM_EINVAL, "tile height must be 8 or 16 rows (got %d)", tile_h);
+    const MultiLayout L = multi_layout(nviews);
+    if (!table || table_bytes < L.total || !staging)
+        return fail(OM_EINVAL, "om_sweep_multi needs a device table and host staging of %zu bytes", L.total);
+    char *host = (char *)staging;
+    SweepK *P = (SweepK *)(host + L.views);
+    int k = 0, r = -1, nf = 0;
+    int *fidx = (int *)(host + L.fidx);
+    int64_t *fpx0 = (int64_t *)(host + L.fpx0);
+    fpx0[0] = 0;
+    for (int v = 0; v < nviews; ++v) {
+        om_sweep_desc d = views[v];
+        if (tile_h)
+            d.tile_h = tile_h;
+        int rc = build_params(&d, P[v]);
+        if (rc)
+            return fail(rc, "view %d: %s", v, om_last_error());
+        if (v == 0) {
+            k = P[v].k;
+            r = P[v].r;
+        } else if (P[v].k != k || P[v].r != r) {
+            return fail(OM_EINVAL, "all views of a step need the same window radius and min(top_k, n)");
+        }
+        SweepK &Q = P[v];
+        Q.hc = std::max(Q.nh, 1);
+        Q.chunks = 1;
+        Q.part = nullptr;
+        Q.depth_out = depth_out ? depth_out[v] : nullptr;
+        if (Q.depth_out) {
+            if (Q.fin_n < 1)
+                return fail(OM_EINVAL, "view %d: the parabola needs at least one hypothesis", v);
+            fidx[nf] = v;
+            fpx0[nf + 1] = fpx0[nf] + (int64_t)Q.H * Q.W;
+            ++nf;
+        }
+    }
+    int *vidx = (int *)(host + L.vidx), *item0 = (int *)(host + L.item0);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    SweepGroup groups[2];
+    int ng = 0, nv = 0;
+    for (int th : {16, 8}) {
+        SweepGroup &g = groups[ng];
+        g.th = th;
+        g.first = nv;
+        int64_t total = 0;
+        for (int v = 0; v < nviews; ++v) {
+            if (P[v].TH != th || P[v].nh < 1)
+                continue;
+            vidx[nv] = v;
+            item0[nv] = (int)total;
+            ++nv;
+            total += (int64_t)P[v].tiles_x * P[v].tiles_y;
+            if (total >= INT_MAX)
+                return fail(OM_EUNSUP, "step has too many work items");
+        }
+        g.nv = nv - g.first;
+        if (g.nv == 0 || total == 0)
+            continue;
+        g.total = (int)total;
+        g.lds = sweep_lds_bytes(th, r);
+        int per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th), kThreads, g.lds) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        g.grid = (int)std::min<int64_t>((int64_t)per_cu * cus, total);
+        ++ng;
+    }
+    hipStream_t st = as_stream(stream);
+    char *dtab = (char *)table;
+    int rc = hip_check(hipMemcpyAsync(dtab, host, L.counter, hipMemcpyHostToDevice, st), "table upload");
+    if (rc)
+        return rc;
+    const SweepK *dviews = (const SweepK *)(dtab + L.views);
+    int *counter = (int *)(dtab + L.counter);
+    ProfScope ps("sweep_multi", st);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counter, 2);
+    for (int gi = 0; gi < ng; ++gi) {
+        const SweepGroup &g = groups[gi];
+        hipLaunchKernelGGL(batch_kernel(k, g.th), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st, dviews,
+                           (const int *)(dtab + L.vidx) + g.first, (const int *)(dtab + L.item0) + g.first, g.nv,
+                           g.total, counter + gi);
+    }
+    if ((rc = launch_check("sweep_batch_kernel")))
+        return rc;
+    if (nf) {
+        hipLaunchKernelGGL(sweep_multi_finalize_kernel, dim3((unsigned)cdiv(fpx0[nf], 256)), dim3(256), 0, st, dviews,
+                           (const int *)(dtab + L.fidx), (const int64_t *)(dtab + L.fpx0), nf);
+        rc = launch_check("sweep_multi_finalize_kernel");
+    }
+    return rc;
+}
+

@@ -301,11 +301,13 @@ def survey_cameras(seed: int, nx: int, ny: int, alt: float, sizes: Sequence[Tupl
 
 
 def make_survey(seed: int, cams: Sequence[Camera], counts: Sequence[int], ext, alt: float, device="cpu",
-                noise: float = 0.01, hmax: float = 15.0, n_bumps: int = 40) -> Survey:
+                noise: float = 0.01, hmax: float = 15.0, n_bumps: int = 40, march: int = 96,
+                bisect: int = 40) -> Survey:
     terr = make_terrain(seed, ext, n_bumps=n_bumps, hmax=hmax)
     views = []
     for k, cam in enumerate(cams):
-        img, dep = render(terr, cam, noise_seed=seed * 1000 + 2 + k, noise=noise, device=device)
+        img, dep = render(terr, cam, noise_seed=seed * 1000 + 2 + k, noise=noise, device=device, march=march,
+                          bisect=bisect)
         views.append(View(cam, img, dep))
     C = np.stack([c.C for c in cams])
     nbrs = neighbours_by_distance(C, alt, counts)
@@ -318,3 +320,20 @@ def config_c3(device="cpu", scale: float = 1.0) -> Survey:
     sizes = [(int(round(w * scale)), int(round(h * scale))) for w, h in C3_SIZES]
     cams, counts, ext = survey_cameras(4, 8, 6, 90.0, sizes, 0.9375, 0.7, 0.6, (4, 12))
     return make_survey(4, cams, counts, ext, 90.0, device=device)
+
+
+C5_CLASSES = ((1200, 900, 0.6), (2400, 1800, 0.3), (4000, 3000, 0.1))
+
+
+def config_c5_pools(device="cpu", scale: float = 1.0, nx: int = 4, ny: int = 4) -> List[Survey]:
+    """Served depth maps (seed 6): one nadir block per image size class (1200x900, 2400x1800,
+    4000x3000; f = 0.9375 W, 90 m, 70/60 % overlap), nx x ny views each, from which requests draw a
+    reference and its nearest views.  A lighter terrain (16 bumps, 64 march + 30 bisection steps)
+    keeps the render of the 280 MP of pools short."""
+    out = []
+    for c, (w, h, _) in enumerate(C5_CLASSES):
+        size = (int(round(w * scale)), int(round(h * scale)))
+        cams, counts, ext = survey_cameras(6 + c, nx, ny, 90.0, [size], 0.9375, 0.7, 0.6, (12, 12))
+        out.append(make_survey(6 + c, cams, counts, ext, 90.0, device=device, n_bumps=16, march=64, bisect=30))
+    return out
+
