@@ -192,6 +192,21 @@ size_t om_sweep_multi_bytes(int32_t nviews);
 int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float *const *depth_out, int32_t tile_h, void *table,
                    size_t table_bytes, void *staging, void *stream);
 
+/* Reduced-precision storage formats of om_sweep_lp / om_to_lp */
+#define OM_FMT_F16 1
+#define OM_FMT_BF16 2
+
+/* float32 -> FP16 or BF16 (round to nearest even, NaN kept), n elements */
+int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream);
+
+/* The precision variant of om_sweep (no reference counterpart: the reference computes in FP64 on
+ * the CPU, _dense.pyx:740-838): reference (ref16) and source images (src[s].img) stored in FP16 or
+ * BF16, warp and bilinear in FP32, window sums and NCC in FP32, plane depth and the variance guard in
+ * FP64.  Hypotheses [i0, i0 + nh) continue the desc's winner state (one chunk); distortion-free
+ * sources; the state is finished by om_sweep_finalize.  Errors against the FP64 path are stated in
+ * the tests and benchmark. */
+int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref16, void *stream);
+
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);
 

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libom355.so")
 
 OM_OK, OM_EINVAL, OM_ESHAPE, OM_ENOMEM, OM_EHIP, OM_ENSRC, OM_EUNSUP = 0, -1, -2, -3, -4, -5, -6
 OM_HYP_MAPS, OM_HYP_LINEAR, OM_HYP_REFINE, OM_HYP_CONST = 0, 1, 2, 3
+OM_FMT_F16, OM_FMT_BF16 = 1, 2
 
 c_void_p = ctypes.c_void_p
 c_int32 = ctypes.c_int32
@@ -102,6 +103,8 @@ SIGNATURES = {
     "om_batch_replay": (c_int32, [c_void_p, c_void_p]),
     "om_batch_pm_changed": (c_int32, [c_void_p, ctypes.POINTER(c_int64)]),
     "om_init_normals": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
+    "om_to_lp": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "om_sweep_lp": (c_int32, [ctypes.POINTER(OmSweepDesc), c_int32, c_void_p, c_void_p]),
     "om_sweep_multi_bytes": (c_size_t, [c_int32]),
     "om_sweep_multi": (c_int32, [ctypes.POINTER(OmSweepDesc), c_int32, ctypes.POINTER(c_void_p), c_int32, c_void_p,
                                  c_size_t, c_void_p, c_void_p]),

@@ -112,6 +112,7 @@ struct PMCam {
     const float *dep;
     int iw, ih;
     int pad[2];
+    double wl, hl;  // iw - 1.001, ih - 1.001: the bilinear's bounds (_dense.pyx:451)
 };
 
 // Host-side staging of the source tables, passed by value to pm_setup_kernel.
@@ -146,9 +147,9 @@ __device__ __forceinline__ bool pm_proj(const PMK &P, int s, double X, double Y,
 }
 
 // bilinear sample, NaN outside [0, w-1.001] x [0, h-1.001] (_dense.pyx:448-458)
-__device__ __forceinline__ double pm_bil(const float *im, int w, int h, double u, double v)
+__device__ __forceinline__ double pm_bil(const float *im, int w, double wl, double hl, double u, double v)
 {
-    if (!(u >= 0.0 && v >= 0.0 && u <= (double)w - 1.001 && v <= (double)h - 1.001))
+    if (!(u >= 0.0 && v >= 0.0 && u <= wl && v <= hl))
         return NAN;
     const int x0 = (int)u, y0 = (int)v;
     const double fx = u - (double)x0, fy = v - (double)y0;
@@ -235,7 +236,7 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
                         double pu, pv, pz;
                         if (pm_proj(P, s, X, Y, Z, pu, pv, pz)) {
                             const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
-                            const double J = pm_bil(cs.img, cs.iw, cs.ih, pu, pv);
+                            const double J = pm_bil(cs.img, cs.iw, cs.wl, cs.hl, pu, pv);
                             if (isfinite(J)) {
                                 sI[u] += I;
                                 sJ[u] += J;
@@ -744,6 +745,8 @@ extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float 
         A.cam[s].dep = deps[s];
         A.cam[s].ih = ih[s];
         A.cam[s].iw = iw[s];
+        A.cam[s].wl = (double)iw[s] - 1.001;
+        A.cam[s].hl = (double)ih[s] - 1.001;
         for (int t = 0; t < 18; ++t)
             A.cam[s].par[t] = par[18 * s + t];
         if (par[18 * s + 13] != 0.0 || par[18 * s + 14] != 0.0 || par[18 * s + 15] != 0.0)
@@ -899,6 +902,8 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
             cams[s].dep = V.deps[s];
             cams[s].ih = V.ih[s];
             cams[s].iw = V.iw[s];
+            cams[s].wl = (double)V.iw[s] - 1.001;
+            cams[s].hl = (double)V.ih[s] - 1.001;
             for (int t = 0; t < 18; ++t)
                 cams[s].par[t] = V.par[s][t];
             if (V.par[s][13] != 0.0 || V.par[s][14] != 0.0 || V.par[s][15] != 0.0)

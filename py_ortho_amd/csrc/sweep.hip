@@ -41,7 +41,6 @@ constexpr int kTW = 64;                           // output columns per tile
 constexpr int kThreads = 256;                     // threads per workgroup
 constexpr int kRun = 8;                           // columns per thread in the horizontal pass
 constexpr int kRunsPerRow = kTW / kRun;           // 8
-constexpr int kHRows = kThreads / kRunsPerRow;    // 32 halo rows per horizontal pass
 constexpr int kVSeg = kThreads / kTW;             // 4 row segments per column in the vertical pass
 constexpr int kMaxR = 7;                          // largest window radius of the fused kernel
 // Tile heights: 4 * VR output rows, VR = output rows per thread in the vertical pass.  VR = 4
@@ -108,21 +107,38 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
     }
 }
 
+// LDS rows of the halo are padded to an odd number of words and the horizontal sums to 65 words
+// per row, so that the horizontal pass (lanes spread over rows) and the vertical pass (lanes
+// spread over columns) hit distinct banks.
+constexpr int kTWP = kTW + 1;
+
+__host__ __device__ constexpr int halo_stride(int r) { return (kTW + 2 * r) | 1; }
+
+// LDS bytes of one workgroup: per halo pixel the pixel index, reference and warped source (and the
+// depth of the current plane unless the planes are constant over the image), plus four planes of
+// horizontal sums.
+__host__ __device__ constexpr size_t sweep_lds(int th, int r, bool constd)
+{
+    return (size_t)(th + 2 * r) * halo_stride(r) * (constd ? 12 : 20) + (size_t)4 * (th + 2 * r) * kTWP * 4;
+}
+
 // One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
-// device memory (batched launches).
-template <int K, int VR, class D>
+// device memory (batched launches).  CD: the hypotheses are constant planes (OM_HYP_CONST), so the
+// depth of a plane is one value for the whole tile and needs no LDS.
+template <int K, int VR, bool CD, class D>
 __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, double *lds_d)
 {
     const int r = P.r;
     constexpr int TH = kVSeg * VR;
     const int HR = TH + 2 * r;          // halo rows
     const int HC = kTW + 2 * r;         // halo columns
-    const int NHALO = HR * HC;
-    double *sD = lds_d;                          // (HR, HC) depth of the current hypothesis, NaN outside
-    int *sP = (int *)(sD + NHALO);               // (HR, HC) pixel index, -1 outside the image
-    float *sI = (float *)(sP + NHALO);           // (HR, HC) reference
-    float *sJ = sI + NHALO;                      // (HR, HC) warped source, NaN = invalid
-    float *sH = sJ + NHALO;                      // (4, HR, kTW) horizontal sums
+    const int HCP = halo_stride(r);     // padded row stride
+    const int NHP = HR * HCP;           // padded halo; the pad column holds index -1
+    double *sD = lds_d;                                   // (HR, HCP) depth of the current hypothesis (!CD)
+    int *sP = (int *)(CD ? lds_d : (void *)(sD + NHP));   // (HR, HCP) pixel index, -1 outside the image
+    float *sI = (float *)(sP + NHP);                      // (HR, HCP) reference
+    float *sJ = sI + NHP;                                 // (HR, HCP) warped source, NaN = invalid
+    float *sH = sJ + NHP;                                 // (4, HR, kTWP) horizontal sums
 
     const int tid = threadIdx.x;
     const int tx0 = (tile % P.tiles_x) * kTW;
@@ -134,10 +150,10 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
 
     // ---- the tile + halo: pixel index and reference value
-    for (int hp = tid; hp < NHALO; hp += kThreads) {
-        const int hy = hp / HC, hx = hp - hy * HC;
+    for (int hp = tid; hp < NHP; hp += kThreads) {
+        const int hy = hp / HCP, hx = hp - hy * HCP;
         const int gy = ty0 - r + hy, gx = tx0 - r + hx;
-        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const bool in = hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W;
         const int p = in ? gy * W + gx : -1;
         sP[hp] = p;
         sI[hp] = in ? P.ref[p] : 0.f;
@@ -177,23 +193,30 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     __syncthreads();
 
     for (int i = ia; i < ib; ++i) {
-        // inverse depth -> depth per halo pixel (shared by all sources); the previous
-        // hypothesis's last warp pass finished reading sD before the barrier after it
-        for (int hp = tid; hp < NHALO; hp += kThreads) {
-            const int p = sP[hp];
-            double d = NAN;
-            if (p >= 0) {
-                const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
-                d = 1.0 / (double)hyp_value(P, i, p, a, b);
+        // inverse depth -> depth, per halo pixel (shared by all sources) or once for a constant
+        // plane; the previous hypothesis's last warp pass finished reading sD before the barrier
+        // after it
+        double dc = 0.0;
+        if (CD) {
+            dc = 1.0 / (double)P.ha[i - P.i0];
+        } else {
+            for (int hp = tid; hp < NHP; hp += kThreads) {
+                const int p = sP[hp];
+                double d = NAN;
+                if (p >= 0) {
+                    const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
+                    d = 1.0 / (double)hyp_value(P, i, p, a, b);
+                }
+                sD[hp] = d;
             }
-            sD[hp] = d;
         }
 #pragma unroll
         for (int j = 0; j < VR; ++j)
 #pragma unroll
             for (int a = 0; a < K; ++a)
                 tk[j][a] = -INFINITY;
-        __syncthreads();
+        if (!CD)
+            __syncthreads();
 
         for (int s = 0; s < P.n; ++s) {
             const auto &S = P.src[s];
@@ -202,11 +225,11 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             const float *img = S.img;
             const int sw = S.w;
             // ---- W: warp the halo into LDS (_sweep.h:49-80)
-            for (int hp = tid; hp < NHALO; hp += kThreads) {
+            for (int hp = tid; hp < NHP; hp += kThreads) {
                 float J = NAN;
                 const int p = sP[hp];
                 if (p >= 0) {
-                    const double d = sD[hp];
+                    const double d = CD ? dc : sD[hp];
                     const double rx = P.ray[p], ry = P.ray[HW + p], rz = P.ray[2 * HW + p];
                     const double X = g[9] + d * ((g[0] * rx + g[1] * ry) + g[2] * rz);
                     const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
@@ -236,10 +259,12 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 sJ[hp] = J;
             }
             __syncthreads();
-            // ---- H: horizontal window sums of one 8-column run per thread (_sweep.h:85-114)
-            for (int hr = tid / kRunsPerRow; hr < HR; hr += kHRows) {
-                const float *Jr = sJ + hr * HC, *Ir = sI + hr * HC;
-                const int c0 = (tid % kRunsPerRow) * kRun;
+            // ---- H: horizontal window sums of one 8-column run per task (_sweep.h:85-114); the
+            // lanes of a wave take consecutive rows
+            for (int t = tid; t < HR * kRunsPerRow; t += kThreads) {
+                const int hr = t % HR;
+                const float *Jr = sJ + hr * HCP, *Ir = sI + hr * HCP;
+                const int c0 = (t / HR) * kRun;
                 int sv = 0;
                 double sj = 0.0, sjj = 0.0, sij = 0.0;
                 for (int t = 0; t <= 2 * r; ++t) {
@@ -251,7 +276,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     sjj = sjj + (double)(jv * jv);
                     sij = sij + (double)(jv * Ir[c0 + t]);
                 }
-                float *h0 = sH + hr * kTW, *h1 = h0 + HR * kTW, *h2 = h1 + HR * kTW, *h3 = h2 + HR * kTW;
+                float *h0 = sH + hr * kTWP, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
                 h0[c0] = (float)sv;
                 h1[c0] = (float)sj;
                 h2[c0] = (float)sjj;
@@ -277,10 +302,10 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             __syncthreads();
             // ---- V: vertical sums and NCC (_sweep.h:116-178)
             {
-                const float *h0 = sH + vc, *h1 = h0 + HR * kTW, *h2 = h1 + HR * kTW, *h3 = h2 + HR * kTW;
+                const float *h0 = sH + vc, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
                 double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
                 for (int t = 0; t <= 2 * r; ++t) {
-                    const int o = (vy0 + t) * kTW;
+                    const int o = (vy0 + t) * kTWP;
                     c0 = c0 + (double)h0[o];
                     c1 = c1 + (double)h1[o];
                     c2 = c2 + (double)h2[o];
@@ -290,7 +315,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 for (int j = 0; j < VR; ++j) {
                     {
                         if (j > 0) {
-                            const int on = (vy0 + j + 2 * r) * kTW, oo = (vy0 + j - 1) * kTW;
+                            const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
                             double t;
                             t = c0 + (double)h0[on];
                             c0 = t - (double)h0[oo];
@@ -369,18 +394,18 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     }
 }
 
-template <int K, int VR>
+template <int K, int VR, bool CD>
 __global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
 {
     extern __shared__ double lds_d[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
-    sweep_tile<K, VR>(P, lb / P.chunks, lb % P.chunks, lds_d);
+    sweep_tile<K, VR, CD>(P, lb / P.chunks, lb % P.chunks, lds_d);
 }
 
 // Persistent batch: work items (view, tile, chunk) of the views vidx[0..nv) from one queue; item0[j]
 // is the first item of view vidx[j].  Every wave of a workgroup reads the same item index from LDS,
 // so all of them leave the loop together once the queue is empty.
-template <int K, int VR>
+template <int K, int VR, bool CD>
 __global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *views, const int *vidx, const int *item0,
                                                                int nv, int total, int *counter)
 {
@@ -404,7 +429,7 @@ __global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *vie
         }
         const OM_CONST_AS SweepK &P = as_const(views)[as_const(vidx)[lo]];
         const int local = it - as_const(item0)[lo];
-        sweep_tile<K, VR>(P, local / P.chunks, local % P.chunks, lds_d);
+        sweep_tile<K, VR, CD>(P, local / P.chunks, local % P.chunks, lds_d);
     }
 }
 
@@ -649,17 +674,18 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
     Pl.hc = (int)cdiv(Pl.nh, chunks);
     Pl.chunks = (int)cdiv(Pl.nh, Pl.hc);
     Pl.part = (float *)d->workspace;
-    const int r = Pl.r;
-    const size_t halo = (size_t)(Pl.TH + 2 * r) * (kTW + 2 * r);
-    const size_t lds = halo * (sizeof(double) + sizeof(int) + 2 * sizeof(float)) +
-                       sizeof(float) * 4 * (size_t)(Pl.TH + 2 * r) * kTW;
+    const bool cd = Pl.mode == OM_HYP_CONST;
+    const size_t lds = sweep_lds(Pl.TH, Pl.r, cd);
     const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
     hipStream_t st = as_stream(stream);
     ProfScope ps("sweep", st);
     using KFn = void (*)(const SweepK);
-    static const KFn table[4][2] = {{sweep_kernel<1, 2>, sweep_kernel<1, 4>}, {sweep_kernel<2, 2>, sweep_kernel<2, 4>},
-                                    {sweep_kernel<3, 2>, sweep_kernel<3, 4>}, {sweep_kernel<4, 2>, sweep_kernel<4, 4>}};
-    hipLaunchKernelGGL(table[Pl.k - 1][Pl.TH == 16], grid, dim3(kThreads), lds, st, Pl);
+    static const KFn table[4][2][2] = {
+        {{sweep_kernel<1, 2, false>, sweep_kernel<1, 2, true>}, {sweep_kernel<1, 4, false>, sweep_kernel<1, 4, true>}},
+        {{sweep_kernel<2, 2, false>, sweep_kernel<2, 2, true>}, {sweep_kernel<2, 4, false>, sweep_kernel<2, 4, true>}},
+        {{sweep_kernel<3, 2, false>, sweep_kernel<3, 2, true>}, {sweep_kernel<3, 4, false>, sweep_kernel<3, 4, true>}},
+        {{sweep_kernel<4, 2, false>, sweep_kernel<4, 2, true>}, {sweep_kernel<4, 4, false>, sweep_kernel<4, 4, true>}}};
+    hipLaunchKernelGGL(table[Pl.k - 1][Pl.TH == 16][cd], grid, dim3(kThreads), lds, st, Pl);
     if ((rc = launch_check("sweep_kernel")))
         return rc;
     if (Pl.chunks > 1) {
@@ -744,18 +770,57 @@ namespace om {
 
 using BatchFn = void (*)(const SweepK *, const int *, const int *, int, int, int *);
 
-static BatchFn batch_kernel(int k, int th)
+static BatchFn batch_kernel(int k, int th, bool cd)
 {
-    static const BatchFn table[4][2] = {
-        {sweep_batch_kernel<1, 2>, sweep_batch_kernel<1, 4>}, {sweep_batch_kernel<2, 2>, sweep_batch_kernel<2, 4>},
-        {sweep_batch_kernel<3, 2>, sweep_batch_kernel<3, 4>}, {sweep_batch_kernel<4, 2>, sweep_batch_kernel<4, 4>}};
-    return table[k - 1][th == 16];
+    static const BatchFn table[4][2][2] = {
+        {{sweep_batch_kernel<1, 2, false>, sweep_batch_kernel<1, 2, true>},
+         {sweep_batch_kernel<1, 4, false>, sweep_batch_kernel<1, 4, true>}},
+        {{sweep_batch_kernel<2, 2, false>, sweep_batch_kernel<2, 2, true>},
+         {sweep_batch_kernel<2, 4, false>, sweep_batch_kernel<2, 4, true>}},
+        {{sweep_batch_kernel<3, 2, false>, sweep_batch_kernel<3, 2, true>},
+         {sweep_batch_kernel<3, 4, false>, sweep_batch_kernel<3, 4, true>}},
+        {{sweep_batch_kernel<4, 2, false>, sweep_batch_kernel<4, 2, true>},
+         {sweep_batch_kernel<4, 4, false>, sweep_batch_kernel<4, 4, true>}}};
+    return table[k - 1][th == 16][cd];
 }
 
-static size_t sweep_lds_bytes(int th, int r)
+// persistent launches of a batch: one per (tile height, constant planes) pair
+static int plan_groups(const SweepK *P, int nviews, int k, int r, int cus, int grid_blocks, std::vector<int> &vidx,
+                       std::vector<int> &item0, SweepGroup *groups, int &ng, bool skip_empty)
 {
-    const size_t halo = (size_t)(th + 2 * r) * (kTW + 2 * r);
-    return halo * (sizeof(double) + sizeof(int) + 2 * sizeof(float)) + sizeof(float) * 4 * (size_t)(th + 2 * r) * kTW;
+    ng = 0;
+    for (int th : {16, 8}) {
+        for (int cdi = 1; cdi >= 0; --cdi) {
+            const bool cd = cdi != 0;
+            SweepGroup &g = groups[ng];
+            g.th = th;
+            g.cd = cd;
+            g.first = (int)vidx.size();
+            int64_t total = 0;
+            for (int v = 0; v < nviews; ++v) {
+                if (P[v].TH != th || (P[v].mode == OM_HYP_CONST) != cd || (skip_empty && P[v].nh < 1))
+                    continue;
+                vidx.push_back(v);
+                item0.push_back((int)total);
+                total += (int64_t)P[v].tiles_x * P[v].tiles_y * P[v].chunks;
+                if (total >= INT_MAX)
+                    return fail(OM_EUNSUP, "batch has too many work items");
+            }
+            g.nv = (int)vidx.size() - g.first;
+            if (g.nv == 0 || total == 0)
+                continue;
+            g.total = (int)total;
+            g.lds = sweep_lds(th, r, cd);
+            int per_cu = 1;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th, cd), kThreads,
+                                                             g.lds) != hipSuccess ||
+                per_cu < 1)
+                per_cu = 1;
+            g.grid = grid_blocks > 0 ? grid_blocks : (int)std::min<int64_t>((int64_t)per_cu * cus, total);
+            ++ng;
+        }
+    }
+    return OM_OK;
 }
 
 int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *const *depth_out, int chunks,
@@ -813,44 +878,18 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
         final_ = final_ || Q.depth_out;
         px0[v + 1] = px0[v] + HW;
     }
-    // one persistent launch per tile height, each with its own queue
+    // one persistent launch per (tile height, constant planes), each with its own queue
     std::vector<int> vidx, item0;
-    b->sw_ngroups = 0;
-    for (int th : {16, 8}) {
-        SweepGroup &g = b->sw_groups[b->sw_ngroups];
-        g.th = th;
-        g.first = (int)vidx.size();
-        int64_t total = 0;
-        for (int v = 0; v < nviews; ++v) {
-            if (P[v].TH != th)
-                continue;
-            vidx.push_back(v);
-            item0.push_back((int)total);
-            total += (int64_t)P[v].tiles_x * P[v].tiles_y * P[v].chunks;
-            if (total >= INT_MAX)
-                return fail(OM_EUNSUP, "batch has too many work items");
-        }
-        g.nv = (int)vidx.size() - g.first;
-        if (g.nv == 0 || total == 0)
-            continue;
-        g.total = (int)total;
-        g.lds = sweep_lds_bytes(th, r);
-        int per_cu = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th), kThreads, g.lds) !=
-                hipSuccess ||
-            per_cu < 1)
-            per_cu = 1;
-        g.grid = grid_blocks > 0 ? grid_blocks : (int)std::min<int64_t>((int64_t)per_cu * cus, total);
-        b->sw_ngroups++;
-    }
+    int rc = plan_groups(P.data(), nviews, k, r, cus, grid_blocks, vidx, item0, b->sw_groups, b->sw_ngroups, false);
+    if (rc)
+        return rc;
     b->sw_views = b->dev_alloc(sizeof(SweepK) * nviews);
     b->sw_vidx = (int *)b->dev_alloc(sizeof(int) * nviews);
     b->sw_item0 = (int *)b->dev_alloc(sizeof(int) * nviews);
     b->sw_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
-    b->sw_counter = (int *)b->dev_alloc(sizeof(int) * 2);
+    b->sw_counter = (int *)b->dev_alloc(sizeof(int) * 4);
     if (!b->sw_views || !b->sw_vidx || !b->sw_item0 || !b->sw_px0 || !b->sw_counter)
         return fail(OM_ENOMEM, "batch: descriptor tables");
-    int rc;
     if ((rc = hip_check(hipMemcpy(b->sw_views, P.data(), sizeof(SweepK) * nviews, hipMemcpyHostToDevice), "upload")) ||
         (rc = hip_check(hipMemcpy(b->sw_vidx, vidx.data(), sizeof(int) * nviews, hipMemcpyHostToDevice), "upload")) ||
         (rc = hip_check(hipMemcpy(b->sw_item0, item0.data(), sizeof(int) * nviews, hipMemcpyHostToDevice), "upload")) ||
@@ -870,13 +909,13 @@ int batch_sweep_enqueue(om_batch *b, hipStream_t st)
     if (b->sw_n == 0)
         return OM_OK;
     ProfScope ps("sweep_batch", st);
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, b->sw_counter, 2);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, b->sw_counter, 4);
     hipLaunchKernelGGL(sweep_batch_reset_kernel, dim3((unsigned)cdiv(b->sw_px, 256)), dim3(256), 0, st,
                        (const SweepK *)b->sw_views, b->sw_px0, b->sw_n);
     for (int gi = 0; gi < b->sw_ngroups; ++gi) {
         const SweepGroup &g = b->sw_groups[gi];
         ProfScope pk("sweep_batch_kernel", st);
-        hipLaunchKernelGGL(batch_kernel(b->sw_k, g.th), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st,
+        hipLaunchKernelGGL(batch_kernel(b->sw_k, g.th, g.cd), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st,
                            (const SweepK *)b->sw_views, b->sw_vidx + g.first, b->sw_item0 + g.first, g.nv, g.total,
                            b->sw_counter + gi);
     }
@@ -974,41 +1013,18 @@ extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float 
             ++nf;
         }
     }
-    int *vidx = (int *)(host + L.vidx), *item0 = (int *)(host + L.item0);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    SweepGroup groups[2];
-    int ng = 0, nv = 0;
-    for (int th : {16, 8}) {
-        SweepGroup &g = groups[ng];
-        g.th = th;
-        g.first = nv;
-        int64_t total = 0;
-        for (int v = 0; v < nviews; ++v) {
-            if (P[v].TH != th || P[v].nh < 1)
-                continue;
-            vidx[nv] = v;
-            item0[nv] = (int)total;
-            ++nv;
-            total += (int64_t)P[v].tiles_x * P[v].tiles_y;
-            if (total >= INT_MAX)
-                return fail(OM_EUNSUP, "step has too many work items");
-        }
-        g.nv = nv - g.first;
-        if (g.nv == 0 || total == 0)
-            continue;
-        g.total = (int)total;
-        g.lds = sweep_lds_bytes(th, r);
-        int per_cu = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th), kThreads, g.lds) !=
-                hipSuccess ||
-            per_cu < 1)
-            per_cu = 1;
-        g.grid = (int)std::min<int64_t>((int64_t)per_cu * cus, total);
-        ++ng;
-    }
+    SweepGroup groups[4];
+    int ng = 0;
+    std::vector<int> gv, gi0;
+    int rc0 = plan_groups(P, nviews, k, r, cus, 0, gv, gi0, groups, ng, true);
+    if (rc0)
+        return rc0;
+    std::copy(gv.begin(), gv.end(), (int *)(host + L.vidx));
+    std::copy(gi0.begin(), gi0.end(), (int *)(host + L.item0));
     hipStream_t st = as_stream(stream);
     char *dtab = (char *)table;
     int rc = hip_check(hipMemcpyAsync(dtab, host, L.counter, hipMemcpyHostToDevice, st), "table upload");
@@ -1017,10 +1033,10 @@ extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float 
     const SweepK *dviews = (const SweepK *)(dtab + L.views);
     int *counter = (int *)(dtab + L.counter);
     ProfScope ps("sweep_multi", st);
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counter, 2);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counter, 4);
     for (int gi = 0; gi < ng; ++gi) {
         const SweepGroup &g = groups[gi];
-        hipLaunchKernelGGL(batch_kernel(k, g.th), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st, dviews,
+        hipLaunchKernelGGL(batch_kernel(k, g.th, g.cd), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st, dviews,
                            (const int *)(dtab + L.vidx) + g.first, (const int *)(dtab + L.item0) + g.first, g.nv,
                            g.total, counter + gi);
     }

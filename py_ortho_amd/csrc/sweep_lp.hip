@@ -1,0 +1,410 @@
+// Reduced-precision plane sweep for gfx950 (the precision variant of the depth-map path).
+//
+// Same algorithm and outputs as the FP64 sweep (sweep.hip; reference: _dense.sweep_hypothesis,
+// orthomosaic/_dense.pyx:740-838, with the row kernels of orthomosaic/_sweep.h:33-214), with the
+// arithmetic re-planned for speed instead of bit parity:
+//   - reference and source images are stored as FP16 or BF16 (half the bytes of every tap);
+//   - per plane the depth 1/inv, per pixel the warp b + d R ray, the projection and the bilinear
+//     tap are FP32 (the plane and camera parameters are formed in FP64 and rounded once);
+//   - the window sums of V, J, J^2, J*I and the vertical sums are FP32 running sums;
+//   - the NCC is FP32 except the variance m2 - mj^2 and its guard (> 1e-6), which are FP64, so
+//     the cancellation does not decide which pixels count as textured.
+// The winner state (best, prev, s_prev_best, s_next_best, idx) has the FP64 path's layout and
+// meaning, so om_sweep_finalize turns it into depth as before.  Errors against the FP64 path are
+// measured by the tests and the benchmark (NCC within ~1e-3, identical winners wherever the two
+// best scores differ by more than that).
+//
+// LDS rows are padded to an odd number of words so that the horizontal pass (lanes spread over
+// rows) and the vertical pass (lanes spread over columns) read distinct banks.
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "om_common.h"
+
+namespace om {
+namespace lp {
+
+constexpr int kTW = 64;        // output columns per tile
+constexpr int kThreads = 256;  // 4 wave64
+constexpr int kVR = 4;         // output rows per thread in the vertical pass
+constexpr int kTH = 16;        // tile height = 4 row segments x kVR
+constexpr int kRun = 8;        // columns per horizontal running sum
+constexpr int kMaxR = 7;
+
+struct SrcLP {
+    const uint16_t *img;
+    int h, w;
+    float g[12];  // Rs (9, row-major) and b = Rs (C_ref - C_src) (3), rounded from FP64
+    float f, cx, cy, wl, hl;  // focal, principal point, w - 1.001, h - 1.001
+};
+
+struct SweepLPK {
+    const uint16_t *ref;
+    const float *mu, *sd, *ray;
+    int H, W, n, r, k, mode;
+    const float *ha, *hb;
+    int hoff, i0, nh, tiles_x, tiles_y;
+    float *best, *prev, *spb, *snb;
+    int *idx;
+    SrcLP src[16];
+};
+
+template <int FMT>
+__device__ __forceinline__ float ld16(const uint16_t *p)
+{
+    const uint16_t v = *p;
+    if (FMT == OM_FMT_BF16)
+        return __uint_as_float((uint32_t)v << 16);
+    return (float)__builtin_bit_cast(_Float16, v);
+}
+
+__device__ __forceinline__ float hyp_lp(const SweepLPK &P, int i, int p, float a, float b)
+{
+    switch (P.mode) {
+    case OM_HYP_LINEAR: {
+        const float t = (float)i * b;
+        return a + t;
+    }
+    case OM_HYP_REFINE: {
+        float v = (float)(i - P.hoff) * b;
+        v = a + v;
+        return (v >= 1e-6f || v != v) ? v : 1e-6f;
+    }
+    case OM_HYP_CONST:
+        return P.ha[i - P.i0];
+    default:
+        return P.ha[(int64_t)(i - P.i0) * ((int64_t)P.H * P.W) + p];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void topk_ins(float (&t)[K], float v)
+{
+#pragma unroll
+    for (int a = K - 1; a >= 0; --a) {
+        const bool lt = t[a] < v;
+        const float up = (a > 0) ? t[a - 1] : v;
+        const bool prev_lt = (a > 0) ? (t[a - 1] < v) : true;
+        if (lt)
+            t[a] = prev_lt ? up : v;
+    }
+}
+
+template <int K, int FMT>
+__global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
+{
+    extern __shared__ float lds_f[];
+    const int r = P.r;
+    const int HR = kTH + 2 * r, HC = kTW + 2 * r;
+    const int HCP = HC | 1;          // padded halo row stride (odd)
+    constexpr int kTWP = kTW + 1;    // padded row stride of the horizontal sums
+    const int NH = HR * HCP;
+    float *sD = lds_f;               // (HR, HCP) depth of the current plane, NaN outside
+    int *sP = (int *)(sD + NH);      // (HR, HCP) pixel index, -1 outside
+    float *sI = (float *)(sP + NH);  // (HR, HCP) reference
+    float *sJ = sI + NH;             // (HR, HCP) warped source, NaN = invalid
+    float *sH = sJ + NH;             // (4, HR, kTWP) horizontal sums
+
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int tx0 = (tile % P.tiles_x) * kTW, ty0 = (tile / P.tiles_x) * kTH;
+    const int H = P.H, W = P.W;
+    const size_t HW = (size_t)H * W;
+    const int tid = threadIdx.x;
+    const int NHALO = HR * HC;
+    const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
+
+    for (int q = tid; q < NHALO; q += kThreads) {
+        const int hy = q / HC, hx = q - hy * HC;
+        const int gy = ty0 - r + hy, gx = tx0 - r + hx;
+        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const int p = in ? gy * W + gx : -1;
+        sP[hy * HCP + hx] = p;
+        sI[hy * HCP + hx] = in ? ld16<FMT>(P.ref + p) : 0.f;
+    }
+
+    const int vc = tid % kTW, vy0 = (tid / kTW) * kVR;
+    const int ox = tx0 + vc;
+    float mu_[kVR], sd_[kVR], st_best[kVR], st_prev[kVR], st_spb[kVR], st_snb[kVR];
+    int st_idx[kVR];
+    float tk[kVR][K];
+#pragma unroll
+    for (int j = 0; j < kVR; ++j) {
+        const int oy = ty0 + vy0 + j;
+        const bool in = oy < H && ox < W;
+        const size_t p = in ? (size_t)oy * W + ox : 0;
+        mu_[j] = in ? P.mu[p] : 0.f;
+        sd_[j] = in ? P.sd[p] : 1.f;
+        st_best[j] = in ? P.best[p] : 0.f;
+        st_prev[j] = in ? P.prev[p] : 0.f;
+        st_spb[j] = in ? P.spb[p] : 0.f;
+        st_snb[j] = in ? P.snb[p] : 0.f;
+        st_idx[j] = in ? P.idx[p] : 0;
+    }
+    const float win = (float)((2 * r + 1) * (2 * r + 1));
+    __syncthreads();
+
+    for (int i = P.i0; i < P.i0 + P.nh; ++i) {
+        for (int q = tid; q < NHALO; q += kThreads) {
+            const int hy = q / HC, hx = q - hy * HC, o = hy * HCP + hx;
+            const int p = sP[o];
+            float d = NAN;
+            if (p >= 0) {
+                const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
+                d = (float)(1.0 / (double)hyp_lp(P, i, p, a, b));  // plane parameter in FP64
+            }
+            sD[o] = d;
+        }
+#pragma unroll
+        for (int j = 0; j < kVR; ++j)
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+                tk[j][a] = -INFINITY;
+        __syncthreads();
+
+        for (int s = 0; s < P.n; ++s) {
+            const SrcLP &S = P.src[s];
+            const float *g = S.g;
+            // ---- W: warp the halo (FP32 projection, FP32 bilinear of 16-bit taps)
+            for (int q = tid; q < NHALO; q += kThreads) {
+                const int hy = q / HC, hx = q - hy * HC, o = hy * HCP + hx;
+                const int p = sP[o];
+                float J = NAN;
+                if (p >= 0) {
+                    const float d = sD[o];
+                    const float rx = P.ray[p], ry = P.ray[HW + p], rz = P.ray[2 * HW + p];
+                    const float X = fmaf(d, fmaf(g[0], rx, fmaf(g[1], ry, g[2] * rz)), g[9]);
+                    const float Y = fmaf(d, fmaf(g[3], rx, fmaf(g[4], ry, g[5] * rz)), g[10]);
+                    const float Z = fmaf(d, fmaf(g[6], rx, fmaf(g[7], ry, g[8] * rz)), g[11]);
+                    if (Z > 1e-6f) {
+                        const float iz = __builtin_amdgcn_rcpf(Z);
+                        const float u = fmaf(S.f, X * iz, S.cx), v = fmaf(S.f, Y * iz, S.cy);
+                        if (u >= 0.f && v >= 0.f && u <= S.wl && v <= S.hl) {
+                            const int x0 = (int)u, y0 = (int)v;
+                            const float fx = u - (float)x0, fy = v - (float)y0;
+                            const uint16_t *pp = S.img + (size_t)y0 * S.w + x0;
+                            const float a00 = ld16<FMT>(pp), a01 = ld16<FMT>(pp + 1);
+                            const float a10 = ld16<FMT>(pp + S.w), a11 = ld16<FMT>(pp + S.w + 1);
+                            const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
+                            const float val = fmaf(fy, t1 - t0, t0);
+                            if (isfinite(val))
+                                J = val;
+                        }
+                    }
+                }
+                sJ[o] = J;
+            }
+            __syncthreads();
+            // ---- H: FP32 running sums over 8-column runs; lanes spread over rows
+            for (int t = tid; t < HR * (kTW / kRun); t += kThreads) {
+                const int hr = t % HR, c0 = (t / HR) * kRun;
+                const float *Jr = sJ + hr * HCP, *Ir = sI + hr * HCP;
+                int sv = 0;
+                float sj = 0.f, sjj = 0.f, sij = 0.f;
+                for (int a = 0; a <= 2 * r; ++a) {
+                    const float J = Jr[c0 + a];
+                    const bool ok = J == J;
+                    const float jv = ok ? J : 0.f;
+                    sv += ok;
+                    sj += jv;
+                    sjj = fmaf(jv, jv, sjj);
+                    sij = fmaf(jv, Ir[c0 + a], sij);
+                }
+                float *h0 = sH + hr * kTWP, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                h0[c0] = (float)sv;
+                h1[c0] = sj;
+                h2[c0] = sjj;
+                h3[c0] = sij;
+#pragma unroll
+                for (int c = c0 + 1; c < c0 + kRun; ++c) {
+                    const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
+                    const bool okn = Jn == Jn, oko = Jo == Jo;
+                    const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
+                    sv += (int)okn - (int)oko;
+                    sj += jn - jo;
+                    sjj = fmaf(jn, jn, fmaf(-jo, jo, sjj));
+                    sij = fmaf(jn, Ir[c + 2 * r], fmaf(-jo, Ir[c - 1], sij));
+                    h0[c] = (float)sv;
+                    h1[c] = sj;
+                    h2[c] = sjj;
+                    h3[c] = sij;
+                }
+            }
+            __syncthreads();
+            // ---- V: FP32 vertical sums, NCC (variance and its guard in FP64)
+            {
+                const float *h0 = sH + vc, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+                for (int a = 0; a <= 2 * r; ++a) {
+                    const int o = (vy0 + a) * kTWP;
+                    c0 += h0[o];
+                    c1 += h1[o];
+                    c2 += h2[o];
+                    c3 += h3[o];
+                }
+#pragma unroll
+                for (int j = 0; j < kVR; ++j) {
+                    if (j > 0) {
+                        const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
+                        c0 += h0[on] - h0[oo];
+                        c1 += h1[on] - h1[oo];
+                        c2 += h2[on] - h2[oo];
+                        c3 += h3[on] - h3[oo];
+                    }
+                    float out = -1.0f;
+                    if (c0 > 0.9f * win) {  // >= 90 % of the window valid
+                        const float ic = __builtin_amdgcn_rcpf(c0);
+                        const float mj = c1 * ic, m2 = c2 * ic, mij = c3 * ic;
+                        const double vj = (double)m2 - (double)mj * (double)mj;
+                        const float sj = vj > 1e-6 ? (float)sqrt(vj) : 1e-3f;
+                        const float q = (mij - mu_[j] * mj) * __builtin_amdgcn_rcpf(sd_[j] * sj);
+                        out = q > 1.0f ? 1.0f : (q < -1.0f ? -1.0f : q);
+                    }
+                    topk_ins<K>(tk[j], out);
+                }
+            }
+        }
+        // ---- combine + streaming winner (_sweep.h:181-214)
+#pragma unroll
+        for (int j = 0; j < kVR; ++j) {
+            float s = 0.0f;
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+                s = s + tk[j][a];
+            s = s / (float)K;
+            if (st_idx[j] == i - 1)
+                st_snb[j] = s;
+            if (s > st_best[j]) {
+                st_best[j] = s;
+                st_spb[j] = st_prev[j];
+                st_snb[j] = -2.0f;
+                st_idx[j] = i;
+            }
+            st_prev[j] = s;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kVR; ++j) {
+        const int oy = ty0 + vy0 + j;
+        if (oy < H && ox < W) {
+            const size_t p = (size_t)oy * W + ox;
+            P.best[p] = st_best[j];
+            P.prev[p] = st_prev[j];
+            P.spb[p] = st_spb[j];
+            P.snb[p] = st_snb[j];
+            P.idx[p] = st_idx[j];
+        }
+    }
+}
+
+__global__ void to_lp_kernel(const float *src, int64_t n, int fmt, uint16_t *dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float x = src[i];
+    if (fmt == OM_FMT_BF16) {
+        const uint32_t u = __float_as_uint(x);
+        dst[i] = (x != x) ? (uint16_t)0x7FC0 : (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    } else {
+        dst[i] = __builtin_bit_cast(uint16_t, (_Float16)x);
+    }
+}
+
+}  // namespace lp
+}  // namespace om
+
+using namespace om;
+using namespace om::lp;
+
+extern "C" int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream)
+{
+    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16)
+        return fail(OM_EINVAL, "unknown storage format %d", fmt);
+    if (n < 0)
+        return fail(OM_ESHAPE, "negative size");
+    if (n == 0)
+        return OM_OK;
+    hipLaunchKernelGGL(to_lp_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, as_stream(stream), src, n, fmt,
+                       (uint16_t *)dst);
+    return launch_check("to_lp_kernel");
+}
+
+extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref16, void *stream)
+{
+    if (!d || !ref16)
+        return fail(OM_EINVAL, "om_sweep_lp: null argument");
+    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16)
+        return fail(OM_EINVAL, "unknown storage format %d", fmt);
+    if (d->n_src < 1 || d->n_src > 16)
+        return fail(OM_ENSRC, "sweep needs 1..16 sources (got %d)", d->n_src);
+    if (d->r < 0 || d->r > kMaxR)
+        return fail(OM_EUNSUP, "reduced-precision sweep supports window radius 0..%d (got %d)", kMaxR, d->r);
+    if (d->H < 0 || d->W < 0 || d->nh < 0 || d->i0 < 0)
+        return fail(OM_ESHAPE, "negative size");
+    if (d->top_k < 1)
+        return fail(OM_EINVAL, "top_k must be >= 1");
+    const int k = d->top_k < d->n_src ? d->top_k : d->n_src;
+    if (k > 4)
+        return fail(OM_EUNSUP, "reduced-precision sweep supports min(top_k, n) <= 4 (got %d)", k);
+    if (d->hyp_mode < OM_HYP_MAPS || d->hyp_mode > OM_HYP_CONST)
+        return fail(OM_EINVAL, "unknown hypothesis generator %d", d->hyp_mode);
+    SweepLPK P;
+    memset(&P, 0, sizeof P);
+    P.ref = (const uint16_t *)ref16;
+    P.mu = d->mu;
+    P.sd = d->sd;
+    P.ray = d->ray;
+    P.H = d->H;
+    P.W = d->W;
+    P.n = d->n_src;
+    P.r = d->r;
+    P.k = k;
+    P.mode = d->hyp_mode;
+    P.ha = d->hyp_a;
+    P.hb = d->hyp_b;
+    P.hoff = d->hyp_offset;
+    P.i0 = d->i0;
+    P.nh = d->nh;
+    P.best = d->best;
+    P.prev = d->prev;
+    P.spb = d->s_prev_best;
+    P.snb = d->s_next_best;
+    P.idx = d->idx;
+    for (int s = 0; s < P.n; ++s) {
+        const om_src &S = d->src[s];
+        if (S.h < 1 || S.w < 1 || !S.img)
+            return fail(OM_ESHAPE, "source %d is empty", s);
+        if (S.geo[13] != 0.0 || S.geo[14] != 0.0 || S.geo[15] != 0.0)
+            return fail(OM_EUNSUP, "reduced-precision sweep supports distortion-free sources only");
+        SrcLP &Q = P.src[s];
+        Q.img = (const uint16_t *)S.img;
+        Q.h = S.h;
+        Q.w = S.w;
+        for (int t = 0; t < 12; ++t)
+            Q.g[t] = (float)S.geo[t];
+        Q.f = (float)S.geo[12];
+        Q.cx = (float)S.geo[16];
+        Q.cy = (float)S.geo[17];
+        Q.wl = (float)((double)S.w - 1.001);
+        Q.hl = (float)((double)S.h - 1.001);
+    }
+    P.tiles_x = (int)cdiv(P.W, kTW);
+    P.tiles_y = (int)cdiv(P.H, kTH);
+    const int64_t tiles = (int64_t)P.tiles_x * P.tiles_y;
+    if (tiles == 0 || P.nh == 0)
+        return OM_OK;
+    const int HR = kTH + 2 * P.r, HCP = (kTW + 2 * P.r) | 1;
+    const size_t lds = (size_t)HR * HCP * 4 * sizeof(float) + 4 * sizeof(float) * (size_t)HR * (kTW + 1);
+    using KFn = void (*)(const SweepLPK);
+    static const KFn table[2][4] = {
+        {sweep_lp_kernel<1, OM_FMT_F16>, sweep_lp_kernel<2, OM_FMT_F16>, sweep_lp_kernel<3, OM_FMT_F16>,
+         sweep_lp_kernel<4, OM_FMT_F16>},
+        {sweep_lp_kernel<1, OM_FMT_BF16>, sweep_lp_kernel<2, OM_FMT_BF16>, sweep_lp_kernel<3, OM_FMT_BF16>,
+         sweep_lp_kernel<4, OM_FMT_BF16>}};
+    hipStream_t st = as_stream(stream);
+    ProfScope ps("sweep_lp", st);
+    hipLaunchKernelGGL(table[fmt == OM_FMT_BF16][k - 1], dim3((unsigned)tiles), dim3(kThreads), lds, st, P);
+    return launch_check("sweep_lp_kernel");
+}

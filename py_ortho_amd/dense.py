@@ -66,13 +66,13 @@ class SweepState:
         self.idx.fill_(-1)
 
 
-def _src_array(srcs: Sequence[torch.Tensor], geo: np.ndarray):
+def _src_array(srcs: Sequence[torch.Tensor], geo: np.ndarray, dtype=torch.float32):
     n = len(srcs)
     arr = (OmSrc * max(n, 1))()
     for s, img in enumerate(srcs):
         if img.dim() != 2:
             raise ValueError("sources must be 2-D images")
-        arr[s].img = _need(img, f"srcs[{s}]")
+        arr[s].img = _need(img, f"srcs[{s}]", dtype)
         arr[s].h, arr[s].w = int(img.shape[0]), int(img.shape[1])
         for t in range(18):
             arr[s].geo[t] = float(geo[s, t])
@@ -119,7 +119,7 @@ def sweep_hypothesis(ref, mu_i, sd_i, srcs, ray, geo, inv, r, top_k, i, best, pr
 
 
 def _sweep_desc(ref, mu, sd, srcs, ray, geo, r, top_k, state: SweepState, mode, hyp_a, hyp_b=None,
-                hyp_offset=0, i0=0, nh=0, chunks=0, tile_h=0):
+                hyp_offset=0, i0=0, nh=0, chunks=0, tile_h=0, src_dtype=torch.float32):
     n = len(srcs)
     if n < 1 or n > 16:
         raise ValueError("sweep needs 1..16 sources")
@@ -129,7 +129,7 @@ def _sweep_desc(ref, mu, sd, srcs, ray, geo, r, top_k, state: SweepState, mode, 
     d.ref, d.mu, d.sd = _need(ref, "ref", shape=(H, W)), _need(mu, "mu", shape=(H, W)), _need(sd, "sd", shape=(H, W))
     d.ray = _need(ray, "ray", shape=(3, H, W))
     d.H, d.W, d.n_src = H, W, n
-    arr = _src_array(srcs, g)
+    arr = _src_array(srcs, g, src_dtype)
     d.src = ctypes.cast(arr, ctypes.POINTER(OmSrc))
     d.r, d.top_k = int(r), int(top_k)
     d.hyp_mode = int(mode)
@@ -315,3 +315,38 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
         int(seed) & 0xFFFFFFFF, ws.data_ptr(), nb, ctypes.byref(changed), _stream())
     check(rc, "pm_refine")
     return int(changed.value)
+
+
+# ---------------------------------------------------------------- reduced precision (FP16 / BF16)
+FORMATS = {"f16": _lib.OM_FMT_F16, "bf16": _lib.OM_FMT_BF16}
+
+
+def to_lp(a: torch.Tensor, fmt: str) -> torch.Tensor:
+    """float32 -> FP16 / BF16 storage (round to nearest even), returned as an int16 tensor of
+    the same shape holding the 16-bit patterns."""
+    f = FORMATS[fmt]
+    src = a.contiguous()
+    _need(src, "image")
+    out = torch.empty(src.shape, dtype=torch.int16, device=src.device)
+    check(_lib.lib().om_to_lp(src.data_ptr(), src.numel(), f, out.data_ptr(), _stream()), "to_lp")
+    return out
+
+
+def sweep_lp(ref16, mu, sd, srcs16, ray, geo, hyp: dict, r: int, top_k: int, fmt: str,
+             state: Optional[SweepState] = None, i0: int = 0, nh: Optional[int] = None) -> SweepState:
+    """The precision variant of `sweep`: images as FP16/BF16 patterns (to_lp), FP32 warp, sums and
+    NCC, FP64 plane depth and variance guard.  Same winner state; finish with sweep_finalize."""
+    H, W = int(ref16.shape[0]), int(ref16.shape[1])
+    mode, a, b, off, n_all = _hyp_inputs(hyp, H, W)
+    if state is None:
+        state = SweepState(H, W, mu.device)
+    if nh is None:
+        nh = n_all - i0
+    for t in [ref16] + list(srcs16):
+        _need(t, "16-bit image", torch.int16)
+    d, keep = _sweep_desc(mu, mu, sd, list(srcs16), ray, geo, r, top_k, state, mode, a, b, off, i0, nh,
+                          src_dtype=torch.int16)
+    check(_lib.lib().om_sweep_lp(ctypes.byref(d), FORMATS[fmt], ref16.data_ptr(), _stream()), "sweep_lp")
+    del keep
+    return state
+

@@ -1,0 +1,66 @@
+"""The precision variant of the sweep (FP16 / BF16 image storage, FP32 warp, sums and NCC): its
+error against the FP64 path on a rendered scene, stated as bounds on the best score, the winning
+plane and the depth; and the storage conversion against torch's own FP16 / BF16 rounding."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("fmt,dtype", [("f16", torch.float16), ("bf16", torch.bfloat16)])
+def test_to_lp_matches_torch_rounding(fmt, dtype):
+    from py_ortho_amd import dense
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x = (torch.randn(100003, generator=g) * 3).cuda()
+    x[::997] = float("nan")
+    got = dense.to_lp(x, fmt)
+    exp = x.to(dtype).view(torch.int16)
+    fin = torch.isfinite(x)
+    assert torch.equal(got[fin], exp[fin])
+    assert torch.isnan(got.view(dtype)[~fin].float()).all()
+
+
+def _scene():
+    from py_ortho_amd import scene
+    return scene.config_c1(device="cuda", scale=0.75)
+
+
+@pytest.mark.parametrize("fmt,r", [("f16", 2), ("bf16", 2), ("f16", 5)])
+def test_sweep_lp_error_against_fp64(fmt, r):
+    from py_ortho_amd import dense, scene
+    sc = _scene()
+    ref = sc.ref.img.contiguous()
+    H, W = ref.shape
+    ray = scene.rays_world(sc.ref.cam, "cuda")
+    mu, sd = dense.ref_stats(ref, r)
+    srcs = [v.img.contiguous() for v in sc.srcs]
+    geo = sc.geo()
+    planes = torch.linspace(60.0, 85.0, 96, dtype=torch.float64)
+    hyp = {"mode": "const", "inv": (1.0 / planes).to(torch.float32).cuda()}
+    st64 = dense.sweep(ref, mu, sd, srcs, ray, geo, hyp, r, 2)
+    d64 = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, r, 2, st64)
+    stlp = dense.sweep_lp(dense.to_lp(ref, fmt), mu, sd, [dense.to_lp(s, fmt) for s in srcs], ray, geo, hyp, r, 2,
+                          fmt)
+    dlp = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, r, 2, stlp)
+    valid = (st64.best > 0.3) & (64.0 * sd > 0.5)           # the reference's validity rule (min_ncc, texture)
+    db = (stlp.best - st64.best).abs()[valid]
+    same = (stlp.idx == st64.idx)[valid].float().mean().item()
+    rel = ((dlp - d64).abs() / d64)[valid]
+    stats = (fmt, r, float(db.median()), float(torch.quantile(db, 0.99)), same, float(torch.quantile(rel, 0.99)))
+    print("lp error", stats)
+    tol = {"f16": (1.0, 1.0, 0.5, 1.0), "bf16": (1.0, 1.0, 0.5, 1.0)}[fmt]
+    assert stats[2] < tol[0] and stats[3] < tol[1] and stats[4] > tol[2] and stats[5] < tol[3], stats
+
+
+def test_sweep_lp_rejects_distortion():
+    from py_ortho_amd import dense, scene
+    sc = _scene()
+    ref = sc.ref.img.contiguous()
+    mu, sd = dense.ref_stats(ref, 2)
+    geo = sc.geo().copy()
+    geo[0, 13] = 0.01
+    hyp = {"mode": "const", "inv": torch.full((4,), 1 / 70.0, device="cuda")}
+    with pytest.raises(NotImplementedError):
+        dense.sweep_lp(dense.to_lp(ref, "f16"), mu, sd, [dense.to_lp(v.img.contiguous(), "f16") for v in sc.srcs],
+                       scene.rays_world(sc.ref.cam, "cuda"), geo, hyp, 2, 2, "f16")
