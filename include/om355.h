@@ -207,6 +207,18 @@ int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream);
  * the tests and benchmark. */
 int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref16, void *stream);
 
+/* The precision variant of om_pm_refine (no reference counterpart: the reference's cost is FP64,
+ * _dense.pyx:461-556): source images imgs16[s] stored as FP16 or BF16 (fmt), the plane (num = d n.ray)
+ * and the geometric term in FP64, per sample the ray-plane intersection, projection and bilinear tap
+ * in FP32, window sums in FP32, the variances and their guard in FP64.  Same red-black order and random
+ * stream as om_pm_refine; distortion-free sources; same workspace. */
+int om_pm_refine_lp(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
+                    const void *const *imgs16, const float *const *deps, const int32_t *ih, const int32_t *iw,
+                    const double *par, float *depth, float *normal, float *score, const uint8_t *active, int32_t r,
+                    int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters, double dmin,
+                    double dmax, uint32_t seed, void *workspace, size_t workspace_bytes, int64_t *changed,
+                    int32_t fmt, void *stream);
+
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);
 

@@ -104,6 +104,12 @@ SIGNATURES = {
     "om_batch_pm_changed": (c_int32, [c_void_p, ctypes.POINTER(c_int64)]),
     "om_init_normals": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
     "om_to_lp": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "om_pm_refine_lp": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, ctypes.POINTER(c_double), c_int32,
+                                  ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_int32),
+                                  ctypes.POINTER(c_int32), ctypes.POINTER(c_double), c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_int32, c_int32, c_int32, c_double, c_double, c_int32, c_double,
+                                  c_double, ctypes.c_uint32, c_void_p, c_size_t, ctypes.POINTER(c_int64), c_int32,
+                                  c_void_p]),
     "om_sweep_lp": (c_int32, [ctypes.POINTER(OmSweepDesc), c_int32, c_void_p, c_void_p]),
     "om_sweep_multi_bytes": (c_size_t, [c_int32]),
     "om_sweep_multi": (c_int32, [ctypes.POINTER(OmSweepDesc), c_int32, ctypes.POINTER(c_void_p), c_int32, c_void_p,

@@ -113,6 +113,9 @@ struct PMCam {
     int iw, ih;
     int pad[2];
     double wl, hl;  // iw - 1.001, ih - 1.001: the bilinear's bounds (_dense.pyx:451)
+    // precision variant only: R (9) and Cr - C (3) rounded to float, f, cx, cy, bounds
+    float rf[9], bf[3], ff, cxf, cyf, wlf, hlf;
+    int pad2;
 };
 
 // Host-side staging of the source tables, passed by value to pm_setup_kernel.
@@ -184,7 +187,112 @@ __device__ __forceinline__ double *pm_ncc_scratch()
 }
 
 // slanted-window cost (_dense.pyx:461-556); photo is written only on the full path
-template <int SM>
+template <int FMT>
+__device__ __forceinline__ float pm_ld16(const uint16_t *p)
+{
+    const uint16_t v = *p;
+    if (FMT == OM_FMT_BF16)
+        return __uint_as_float((uint32_t)v << 16);
+    return (float)__builtin_bit_cast(_Float16, v);
+}
+
+constexpr int kGroupLP = 8;  // sources accumulated together by the precision variant
+
+// Precision variant of the photometric part of pm_cost: the plane (num = d n.ray_p) in FP64,
+// rounded once; per sample the ray-plane intersection, the projection into each source and the
+// bilinear tap of its FP16 / BF16 image in FP32; window sums in FP32; the variances and their
+// 1e-8 guard in FP64.  -> the S NCC values in ncc_l.
+template <int FMT>
+__device__ __forceinline__ void pm_ncc_lp(const PMK &P, int y, int x, double num, double n0, double n1, double n2,
+                                          double *ncc_l)
+{
+    const int H = P.H, W = P.W, r = P.r, st = P.step, S = P.S;
+    const size_t HW = (size_t)H * W;
+    const float numf = (float)num, nf0 = (float)n0, nf1 = (float)n1, nf2 = (float)n2;
+#pragma unroll 1
+    for (int g0 = 0; g0 < S; g0 += kGroupLP) {
+        float sI[kGroupLP], sJ[kGroupLP], sII[kGroupLP], sJJ[kGroupLP], sIJ[kGroupLP];
+        int cnt[kGroupLP];
+#pragma unroll
+        for (int u = 0; u < kGroupLP; ++u) {
+            sI[u] = sJ[u] = sII[u] = sJJ[u] = sIJ[u] = 0.f;
+            cnt[u] = 0;
+        }
+        int nsamp = 0;
+        for (int a = -r; a <= r; a += st) {
+            const int yy = y + a;
+            if (yy < 0 || yy >= H)
+                continue;
+            for (int b = -r; b <= r; b += st) {
+                const int xx = x + b;
+                if (xx < 0 || xx >= W)
+                    continue;
+                const size_t q = (size_t)yy * W + xx;
+                const float I = P.ref[q];
+                if (!isfinite(I))
+                    continue;
+                const float qx = P.ray[q], qy = P.ray[HW + q], qz = P.ray[2 * HW + q];
+                const float den = fmaf(nf0, qx, fmaf(nf1, qy, nf2 * qz));
+                if (den >= -1e-9f)
+                    continue;
+                const float t = numf * __builtin_amdgcn_rcpf(den);
+                nsamp += 1;
+#pragma unroll
+                for (int u = 0; u < kGroupLP; ++u) {
+                    const int s = g0 + u;
+                    if (s < S) {
+                        const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
+                        const float dx = fmaf(t, qx, cs.bf[0]), dy = fmaf(t, qy, cs.bf[1]), dz = fmaf(t, qz, cs.bf[2]);
+                        const float zc = fmaf(cs.rf[6], dx, fmaf(cs.rf[7], dy, cs.rf[8] * dz));
+                        if (zc > 1e-6f) {
+                            const float xc = fmaf(cs.rf[0], dx, fmaf(cs.rf[1], dy, cs.rf[2] * dz));
+                            const float yc = fmaf(cs.rf[3], dx, fmaf(cs.rf[4], dy, cs.rf[5] * dz));
+                            const float iz = __builtin_amdgcn_rcpf(zc);
+                            const float pu = fmaf(cs.ff, xc * iz, cs.cxf), pv = fmaf(cs.ff, yc * iz, cs.cyf);
+                            if (pu >= 0.f && pv >= 0.f && pu <= cs.wlf && pv <= cs.hlf) {
+                                const int x0 = (int)pu, y0 = (int)pv;
+                                const float fx = pu - (float)x0, fy = pv - (float)y0;
+                                const uint16_t *pp = (const uint16_t *)cs.img + (size_t)y0 * cs.iw + x0;
+                                const float a00 = pm_ld16<FMT>(pp), a01 = pm_ld16<FMT>(pp + 1);
+                                const float a10 = pm_ld16<FMT>(pp + cs.iw), a11 = pm_ld16<FMT>(pp + cs.iw + 1);
+                                const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
+                                const float J = fmaf(fy, t1 - t0, t0);
+                                if (isfinite(J)) {
+                                    sI[u] += I;
+                                    sJ[u] += J;
+                                    sII[u] = fmaf(I, I, sII[u]);
+                                    sJJ[u] = fmaf(J, J, sJJ[u]);
+                                    sIJ[u] = fmaf(I, J, sIJ[u]);
+                                    cnt[u] += 1;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kGroupLP; ++u) {
+            const int s = g0 + u;
+            if (s < S) {
+                double v;
+                if (nsamp < 4 || cnt[u] < 0.7 * nsamp) {
+                    v = -1.0;
+                } else {
+                    const float im = __builtin_amdgcn_rcpf((float)cnt[u]);
+                    const float mi = sI[u] * im, mj = sJ[u] * im;
+                    const double vi = (double)(sII[u] * im) - (double)mi * mi;
+                    const double vj = (double)(sJJ[u] * im) - (double)mj * mj;
+                    const float cv = fmaf(-mi, mj, sIJ[u] * im);
+                    v = (vi < 1e-8 || vj < 1e-8) ? 0.0 : (double)(cv * __builtin_amdgcn_rsqf((float)(vi * vj)));
+                }
+                ncc_l[s] = v;
+            }
+        }
+    }
+}
+
+template <int SM, int FMT>
 __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, double n1, double n2, double &photo)
 {
     const int H = P.H, W = P.W;
@@ -196,6 +304,9 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
     const double num = d * ndr;
     const int r = P.r, st = P.step, S = P.S;
     double *ncc_l = pm_ncc_scratch();  // this thread's S NCC values (LDS)
+    if (FMT != 0) {
+        pm_ncc_lp<FMT>(P, y, x, num, n0, n1, n2, ncc_l);
+    } else {
     int nsamp = 0;
     // Sources in groups of kGroup, the group loop kept rolled: per-sample geometry is recomputed
     // once per group, only kGroup sets of accumulators and one group's camera parameters are live.
@@ -268,6 +379,7 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
                 ncc_l[s] = v;
             }
         }
+    }
     }
     double ncc[SM];
 #pragma unroll
@@ -353,7 +465,7 @@ __device__ __forceinline__ void count_changes(const PMK &P, int n)
 }
 
 // A: current plane + propagation from the 8 neighbours (:614-645), pixel t of the colour
-template <int SM>
+template <int SM, int FMT>
 __device__ __forceinline__ void pm_propagate_px(const PMK &P, const PassK &Q, int64_t t)
 {
     static const int oy[8] = {-1, 1, 0, 0, -5, 5, 0, 0};
@@ -369,7 +481,7 @@ __device__ __forceinline__ void pm_propagate_px(const PMK &P, const PassK &Q, in
     double best, sc;
     const float d0 = P.depth[p];
     if (isfinite(d0)) {
-        best = pm_cost<SM>(P, y, x, d0, n0, n1, n2, photo);
+        best = pm_cost<SM, FMT>(P, y, x, d0, n0, n1, n2, photo);
         sc = photo;
     } else {
         best = 1e9;
@@ -392,7 +504,7 @@ __device__ __forceinline__ void pm_propagate_px(const PMK &P, const PassK &Q, in
         const double dp = (double)dq * dn / ndr;
         if (dp < P.dmin || dp > P.dmax)
             continue;
-        const double cst = pm_cost<SM>(P, y, x, dp, b0, b1, b2, photo);
+        const double cst = pm_cost<SM, FMT>(P, y, x, dp, b0, b1, b2, photo);
         if (cst < best) {
             best = cst;
             sc = photo;
@@ -475,7 +587,7 @@ __device__ __forceinline__ void pm_scan_view(const PMK &P, const PassK &Q)
 }
 
 // C: the 2 random refinements with the pixel's own slice of the stream (:646-664); score write
-template <int SM>
+template <int SM, int FMT>
 __device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64_t t)
 {
     int y, x;
@@ -501,7 +613,7 @@ __device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64
             b0 /= nn;
             b1 /= nn;
             b2 /= nn;
-            const double cst = pm_cost<SM>(P, y, x, dp, b0, b1, b2, photo);
+            const double cst = pm_cost<SM, FMT>(P, y, x, dp, b0, b1, b2, photo);
             if (cst < best) {
                 best = cst;
                 sc = photo;
@@ -518,20 +630,20 @@ __device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64
 }
 
 // ---------------------------------------------------------------- single-view kernels
-template <int SM>
+template <int SM, int FMT>
 __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_propagate_kernel(const PMK P, const PassK Q)
 {
-    pm_propagate_px<SM>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    pm_propagate_px<SM, FMT>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(256) pm_rank_rows_kernel(const PMK P, const PassK Q) { pm_rank_row(P, Q, blockIdx.x); }
 
 __global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P, const PassK Q) { pm_scan_view(P, Q); }
 
-template <int SM>
+template <int SM, int FMT>
 __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_kernel(const PMK P, const PassK Q)
 {
-    pm_random_px<SM>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    pm_random_px<SM, FMT>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 __global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed, const PMCamArgs A,
@@ -564,7 +676,7 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_propagate_batch(cons
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int v = view_of(blk0, nviews, b);
-    pm_propagate_px<SM>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
+    pm_propagate_px<SM, 0>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
 }
 
 template <int SM>
@@ -573,7 +685,7 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_batch(const P
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int v = view_of(blk0, nviews, b);
-    pm_random_px<SM>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
+    pm_random_px<SM, 0>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(256) pm_rank_rows_batch(const PMK *views, const int *row0, int nviews,
@@ -632,7 +744,7 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w)
     return o;
 }
 
-template <int SM>
+template <int SM, int FMT>
 static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uint32_t seed, const PMWs &w)
 {
     const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
@@ -650,7 +762,7 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
             Q.nsc = 0.3 / (it + 1);
             {
                 ProfScope ps("pm_propagate", stream);
-                hipLaunchKernelGGL(pm_propagate_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                hipLaunchKernelGGL((pm_propagate_kernel<SM, FMT>), dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
             }
             {
                 ProfScope ps("pm_rank_rows", stream);
@@ -662,7 +774,7 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
             }
             {
                 ProfScope ps("pm_random", stream);
-                hipLaunchKernelGGL(pm_random_kernel<SM>, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                hipLaunchKernelGGL((pm_random_kernel<SM, FMT>), dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
             }
         }
     }
@@ -680,12 +792,12 @@ extern "C" size_t om_pm_workspace_bytes(int32_t H, int32_t W)
     return pm_ws_layout(H, W, nullptr, nullptr);
 }
 
-extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
-                            const float *const *imgs, const float *const *deps, const int32_t *ih, const int32_t *iw,
-                            const double *par, float *depth, float *normal, float *score, const uint8_t *active,
-                            int32_t r, int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters,
-                            double dmin, double dmax, uint32_t seed, void *workspace, size_t workspace_bytes,
-                            int64_t *changed, void *stream)
+static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
+                          const void *const *imgs, const float *const *deps, const int32_t *ih, const int32_t *iw,
+                          const double *par, float *depth, float *normal, float *score, const uint8_t *active,
+                          int32_t r, int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters,
+                          double dmin, double dmax, uint32_t seed, void *workspace, size_t workspace_bytes,
+                          int64_t *changed, void *stream, int fmt)
 {
     if (changed)
         *changed = 0;
@@ -741,27 +853,39 @@ extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float 
     for (int s = 0; s < S; ++s) {
         if (ih[s] < 1 || iw[s] < 1)
             return fail(OM_ESHAPE, "source %d has an empty image", s);
-        A.cam[s].img = imgs[s];
-        A.cam[s].dep = deps[s];
-        A.cam[s].ih = ih[s];
-        A.cam[s].iw = iw[s];
-        A.cam[s].wl = (double)iw[s] - 1.001;
-        A.cam[s].hl = (double)ih[s] - 1.001;
+        PMCam &c = A.cam[s];
+        c.img = (const float *)imgs[s];
+        c.dep = deps[s];
+        c.ih = ih[s];
+        c.iw = iw[s];
+        c.wl = (double)iw[s] - 1.001;
+        c.hl = (double)ih[s] - 1.001;
         for (int t = 0; t < 18; ++t)
-            A.cam[s].par[t] = par[18 * s + t];
+            c.par[t] = par[18 * s + t];
         if (par[18 * s + 13] != 0.0 || par[18 * s + 14] != 0.0 || par[18 * s + 15] != 0.0)
             P.pinhole = 0;
+        for (int t = 0; t < 9; ++t)
+            c.rf[t] = (float)c.par[t];
+        for (int t = 0; t < 3; ++t)
+            c.bf[t] = (float)(Cr[t] - c.par[9 + t]);
+        c.ff = (float)c.par[12];
+        c.cxf = (float)c.par[16];
+        c.cyf = (float)c.par[17];
+        c.wlf = (float)c.wl;
+        c.hlf = (float)c.hl;
     }
+    if (fmt != 0 && !P.pinhole)
+        return fail(OM_EUNSUP, "the precision variant of pm_refine supports distortion-free sources only");
     hipStream_t st = as_stream(stream);
     int rc;
-    if (S <= 4)
-        rc = pm_run<4>(P, A, iters, st, seed, w);
-    else if (S <= 8)
-        rc = pm_run<8>(P, A, iters, st, seed, w);
-    else if (S <= 12)
-        rc = pm_run<12>(P, A, iters, st, seed, w);
-    else
-        rc = pm_run<16>(P, A, iters, st, seed, w);
+    using RunFn = int (*)(PMK &, const PMCamArgs &, int, hipStream_t, uint32_t, const PMWs &);
+    static const RunFn runs[3][4] = {{pm_run<4, 0>, pm_run<8, 0>, pm_run<12, 0>, pm_run<16, 0>},
+                                     {pm_run<4, OM_FMT_F16>, pm_run<8, OM_FMT_F16>, pm_run<12, OM_FMT_F16>,
+                                      pm_run<16, OM_FMT_F16>},
+                                     {pm_run<4, OM_FMT_BF16>, pm_run<8, OM_FMT_BF16>, pm_run<12, OM_FMT_BF16>,
+                                      pm_run<16, OM_FMT_BF16>}};
+    const int bucket = S <= 4 ? 0 : (S <= 8 ? 1 : (S <= 12 ? 2 : 3));
+    rc = runs[fmt][bucket](P, A, iters, st, seed, w);
     if (rc)
         return rc;
     if (changed) {
@@ -773,6 +897,32 @@ extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float 
         *changed = (int64_t)h;
     }
     return OM_OK;
+}
+
+extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
+                            const float *const *imgs, const float *const *deps, const int32_t *ih, const int32_t *iw,
+                            const double *par, float *depth, float *normal, float *score, const uint8_t *active,
+                            int32_t r, int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters,
+                            double dmin, double dmax, uint32_t seed, void *workspace, size_t workspace_bytes,
+                            int64_t *changed, void *stream)
+{
+    return pm_refine_impl(ref, H, W, ray, Cr, S, (const void *const *)imgs, deps, ih, iw, par, depth, normal, score,
+                          active, r, step, top_k, geo_w, geo_tol, iters, dmin, dmax, seed, workspace, workspace_bytes,
+                          changed, stream, 0);
+}
+
+extern "C" int om_pm_refine_lp(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
+                               const void *const *imgs16, const float *const *deps, const int32_t *ih,
+                               const int32_t *iw, const double *par, float *depth, float *normal, float *score,
+                               const uint8_t *active, int32_t r, int32_t step, int32_t top_k, double geo_w,
+                               double geo_tol, int32_t iters, double dmin, double dmax, uint32_t seed, void *workspace,
+                               size_t workspace_bytes, int64_t *changed, int32_t fmt, void *stream)
+{
+    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16)
+        return fail(OM_EINVAL, "unknown storage format %d", fmt);
+    return pm_refine_impl(ref, H, W, ray, Cr, S, imgs16, deps, ih, iw, par, depth, normal, score, active, r, step,
+                          top_k, geo_w, geo_tol, iters, dmin, dmax, seed, workspace, workspace_bytes, changed, stream,
+                          fmt);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -92,7 +92,8 @@ __device__ __forceinline__ void topk_ins(float (&t)[K], float v)
     }
 }
 
-template <int K, int FMT>
+// CD: constant planes (OM_HYP_CONST): one depth per plane, no per-pixel depth in LDS
+template <int K, int FMT, bool CD>
 __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
 {
     extern __shared__ float lds_f[];
@@ -101,8 +102,8 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     const int HCP = HC | 1;          // padded halo row stride (odd)
     constexpr int kTWP = kTW + 1;    // padded row stride of the horizontal sums
     const int NH = HR * HCP;
-    float *sD = lds_f;               // (HR, HCP) depth of the current plane, NaN outside
-    int *sP = (int *)(sD + NH);      // (HR, HCP) pixel index, -1 outside
+    float *sD = lds_f;                             // (HR, HCP) depth of the current plane (!CD)
+    int *sP = (int *)(CD ? lds_f : sD + NH);      // (HR, HCP) pixel index, -1 outside
     float *sI = (float *)(sP + NH);  // (HR, HCP) reference
     float *sJ = sI + NH;             // (HR, HCP) warped source, NaN = invalid
     float *sH = sJ + NH;             // (4, HR, kTWP) horizontal sums
@@ -112,16 +113,16 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     const int H = P.H, W = P.W;
     const size_t HW = (size_t)H * W;
     const int tid = threadIdx.x;
-    const int NHALO = HR * HC;
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
 
-    for (int q = tid; q < NHALO; q += kThreads) {
-        const int hy = q / HC, hx = q - hy * HC;
+    // the padded halo layout: the pad column (when HC is even) holds index -1 like the outside
+    for (int o = tid; o < NH; o += kThreads) {
+        const int hy = o / HCP, hx = o - hy * HCP;
         const int gy = ty0 - r + hy, gx = tx0 - r + hx;
-        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const bool in = hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W;
         const int p = in ? gy * W + gx : -1;
-        sP[hy * HCP + hx] = p;
-        sI[hy * HCP + hx] = in ? ld16<FMT>(P.ref + p) : 0.f;
+        sP[o] = p;
+        sI[o] = in ? ld16<FMT>(P.ref + p) : 0.f;
     }
 
     const int vc = tid % kTW, vy0 = (tid / kTW) * kVR;
@@ -146,33 +147,37 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     __syncthreads();
 
     for (int i = P.i0; i < P.i0 + P.nh; ++i) {
-        for (int q = tid; q < NHALO; q += kThreads) {
-            const int hy = q / HC, hx = q - hy * HC, o = hy * HCP + hx;
-            const int p = sP[o];
-            float d = NAN;
-            if (p >= 0) {
-                const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
-                d = (float)(1.0 / (double)hyp_lp(P, i, p, a, b));  // plane parameter in FP64
+        float dc = 0.f;
+        if (CD) {
+            dc = (float)(1.0 / (double)P.ha[i - P.i0]);  // plane parameter in FP64, rounded once
+        } else {
+            for (int o = tid; o < NH; o += kThreads) {
+                const int p = sP[o];
+                float d = NAN;
+                if (p >= 0) {
+                    const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
+                    d = (float)(1.0 / (double)hyp_lp(P, i, p, a, b));
+                }
+                sD[o] = d;
             }
-            sD[o] = d;
         }
 #pragma unroll
         for (int j = 0; j < kVR; ++j)
 #pragma unroll
             for (int a = 0; a < K; ++a)
                 tk[j][a] = -INFINITY;
-        __syncthreads();
+        if (!CD)
+            __syncthreads();
 
         for (int s = 0; s < P.n; ++s) {
             const SrcLP &S = P.src[s];
             const float *g = S.g;
             // ---- W: warp the halo (FP32 projection, FP32 bilinear of 16-bit taps)
-            for (int q = tid; q < NHALO; q += kThreads) {
-                const int hy = q / HC, hx = q - hy * HC, o = hy * HCP + hx;
+            for (int o = tid; o < NH; o += kThreads) {
                 const int p = sP[o];
                 float J = NAN;
                 if (p >= 0) {
-                    const float d = sD[o];
+                    const float d = CD ? dc : sD[o];
                     const float rx = P.ray[p], ry = P.ray[HW + p], rz = P.ray[2 * HW + p];
                     const float X = fmaf(d, fmaf(g[0], rx, fmaf(g[1], ry, g[2] * rz)), g[9]);
                     const float Y = fmaf(d, fmaf(g[3], rx, fmaf(g[4], ry, g[5] * rz)), g[10]);
@@ -395,16 +400,19 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
     const int64_t tiles = (int64_t)P.tiles_x * P.tiles_y;
     if (tiles == 0 || P.nh == 0)
         return OM_OK;
+    const bool cd = P.mode == OM_HYP_CONST;
     const int HR = kTH + 2 * P.r, HCP = (kTW + 2 * P.r) | 1;
-    const size_t lds = (size_t)HR * HCP * 4 * sizeof(float) + 4 * sizeof(float) * (size_t)HR * (kTW + 1);
+    const size_t lds = (size_t)HR * HCP * (cd ? 3 : 4) * sizeof(float) + 4 * sizeof(float) * (size_t)HR * (kTW + 1);
     using KFn = void (*)(const SweepLPK);
-    static const KFn table[2][4] = {
-        {sweep_lp_kernel<1, OM_FMT_F16>, sweep_lp_kernel<2, OM_FMT_F16>, sweep_lp_kernel<3, OM_FMT_F16>,
-         sweep_lp_kernel<4, OM_FMT_F16>},
-        {sweep_lp_kernel<1, OM_FMT_BF16>, sweep_lp_kernel<2, OM_FMT_BF16>, sweep_lp_kernel<3, OM_FMT_BF16>,
-         sweep_lp_kernel<4, OM_FMT_BF16>}};
+#define OM_LP_ROW(FMT_)                                                                                           \
+    {{sweep_lp_kernel<1, FMT_, false>, sweep_lp_kernel<1, FMT_, true>},                                            \
+     {sweep_lp_kernel<2, FMT_, false>, sweep_lp_kernel<2, FMT_, true>},                                            \
+     {sweep_lp_kernel<3, FMT_, false>, sweep_lp_kernel<3, FMT_, true>},                                            \
+     {sweep_lp_kernel<4, FMT_, false>, sweep_lp_kernel<4, FMT_, true>}}
+    static const KFn table[2][4][2] = {OM_LP_ROW(OM_FMT_F16), OM_LP_ROW(OM_FMT_BF16)};
+#undef OM_LP_ROW
     hipStream_t st = as_stream(stream);
     ProfScope ps("sweep_lp", st);
-    hipLaunchKernelGGL(table[fmt == OM_FMT_BF16][k - 1], dim3((unsigned)tiles), dim3(kThreads), lds, st, P);
+    hipLaunchKernelGGL(table[fmt == OM_FMT_BF16][k - 1][cd], dim3((unsigned)tiles), dim3(kThreads), lds, st, P);
     return launch_check("sweep_lp_kernel");
 }

@@ -278,9 +278,10 @@ def rays_world(H: int, W: int, R, cx: float, cy: float, f: float, k1: float = 0.
 
 
 def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active, r, step, top_k, geo_w, geo_tol,
-              iters, dmin, dmax, seed) -> int:
+              iters, dmin, dmax, seed, fmt: Optional[str] = None) -> int:
     """PatchMatch repair of a depth map (_dense.pyx:566-666): only `active` pixels are re-estimated;
-    depth / normal / score are updated in place.  Returns the number of accepted changes."""
+    depth / normal / score are updated in place.  Returns the number of accepted changes.
+    fmt "f16" / "bf16": the precision variant, with `srcs` as 16-bit patterns (to_lp)."""
     S = len(srcs)
     if S == 0 or S > 16:
         return 0
@@ -293,7 +294,8 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
         raise ValueError("Cr must have 3 values")
     if len(src_depths) != S:
         raise ValueError("src_depths must have one entry (or None) per source")
-    imgs = (ctypes.c_void_p * S)(*[_need(s, f"srcs[{k}]") for k, s in enumerate(srcs)])
+    sdt = torch.float32 if fmt is None else torch.int16
+    imgs = (ctypes.c_void_p * S)(*[_need(s, f"srcs[{k}]", sdt) for k, s in enumerate(srcs)])
     deps = (ctypes.c_void_p * S)(*[None if d is None else _need(d, f"src_depths[{k}]")
                                   for k, d in enumerate(src_depths)])
     for k, d in enumerate(src_depths):
@@ -307,12 +309,15 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
     nb = L.om_pm_workspace_bytes(H, W)
     ws = _workspace(nb, ref.device)
     changed = ctypes.c_int64(0)
-    rc = L.om_pm_refine(
-        _need(ref, "ref", shape=(H, W)), H, W, _need(ray, "ray", shape=(3, H, W)), cd, S, imgs, deps, ih, iw, pd,
-        _need(depth, "depth", shape=(H, W)), _need(normal, "normal", shape=(3, H, W)),
-        _need(score, "score", shape=(H, W)), _need(active, "active", torch.uint8, shape=(H, W)), int(r), int(step),
-        int(top_k), float(geo_w), float(geo_tol), int(iters), float(dmin), float(dmax),
-        int(seed) & 0xFFFFFFFF, ws.data_ptr(), nb, ctypes.byref(changed), _stream())
+    args = (_need(ref, "ref", shape=(H, W)), H, W, _need(ray, "ray", shape=(3, H, W)), cd, S, imgs, deps, ih, iw, pd,
+            _need(depth, "depth", shape=(H, W)), _need(normal, "normal", shape=(3, H, W)),
+            _need(score, "score", shape=(H, W)), _need(active, "active", torch.uint8, shape=(H, W)), int(r),
+            int(step), int(top_k), float(geo_w), float(geo_tol), int(iters), float(dmin), float(dmax),
+            int(seed) & 0xFFFFFFFF, ws.data_ptr(), nb, ctypes.byref(changed))
+    if fmt is None:
+        rc = L.om_pm_refine(*args, _stream())
+    else:
+        rc = L.om_pm_refine_lp(*args, FORMATS[fmt], _stream())
     check(rc, "pm_refine")
     return int(changed.value)
 

@@ -49,7 +49,12 @@ def test_sweep_lp_error_against_fp64(fmt, r):
     rel = ((dlp - d64).abs() / d64)[valid]
     stats = (fmt, r, float(db.median()), float(torch.quantile(db, 0.99)), same, float(torch.quantile(rel, 0.99)))
     print("lp error", stats)
-    tol = {"f16": (1.0, 1.0, 0.5, 1.0), "bf16": (1.0, 1.0, 0.5, 1.0)}[fmt]
+    # stated error bounds (measured on MI355X: f16 r=2 4.9e-3 / 0.050 / 0.991 / 3.7e-4, f16 r=5
+    # 1.2e-3 / 0.011 / 0.997 / 7.1e-5, bf16 r=2 0.038 / 0.32 / 0.893 / 6.6e-3): median and 99th
+    # percentile of |best score - FP64|, share of identical winning planes, 99th percentile of the
+    # relative depth difference, over pixels the reference would keep
+    tol = {("f16", 2): (1e-2, 0.1, 0.98, 1e-3), ("f16", 5): (3e-3, 0.03, 0.99, 3e-4),
+           ("bf16", 2): (0.06, 0.5, 0.85, 2e-2)}[(fmt, r)]
     assert stats[2] < tol[0] and stats[3] < tol[1] and stats[4] > tol[2] and stats[5] < tol[3], stats
 
 
