@@ -279,6 +279,10 @@ def run_c1(args, torch):
         "accuracy": {"valid_frac": float(valid.float().mean()),
                      "valid_within_1pct": float((rel[valid] < 0.01).float().mean())},
     }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_sweep_rate(ref.cpu().numpy(), mu.cpu().numpy(), sd.cpu().numpy(),
+                                             ray.cpu().numpy(), [x.cpu().numpy() for x in srcs], geo,
+                                             inv.cpu().numpy(), r, k, args, f"the {W}x{H} view, S={S}")
     return res
 
 
@@ -541,7 +545,8 @@ def run_c5(args, torch):
     window, top_k = 5, 2
     ex = serve.GpuExecutor(window=window, top_k=top_k)
     store = serve.ImageStore(lambda key: images[key], args.budget_gb * 1e9, upload=ex.upload)
-    srv = serve.DepthServer(store, ex, budget_bytes=args.budget_gb * 1e9, step_planes=args.step_planes).start()
+    srv = serve.DepthServer(store, ex, budget_bytes=args.budget_gb * 1e9, step_planes=args.step_planes,
+                            max_active_pixels=args.active_mpix * 1e6).start()
     # warm-up: two small requests (code objects, allocator)
     warm = c5_requests(pools, 50.0, 0.05, seed=99)[:2]
     for _, r in warm:
@@ -619,7 +624,7 @@ def run_c5(args, torch):
                                "1200x900 / 2400x1800 / 4000x3000 (60/30/10 pct)"
                                f" x {args.scale:g}, S=U{{4..12}}, planes U{{128..384}}, 11x11 NCC top-2 + parabola, "
                                f"{args.budget_gb:g} GB admission budget, continuous batching ({args.step_planes} "
-                               "planes per step)",
+                               f"planes per step, <= {args.active_mpix:g} MP in flight)",
                    "requests": len(arrivals), "pixel_hypotheses": int(sum(px_h)), "wall_s": wall,
                    "offered_mpix_hyp_s": sum(px_h) / args.duration / 1e6, "steps": srv.steps,
                    "max_active": srv.stats["max_active"], "image_uploads": store.uploads, "scene_s": t_scene},
@@ -642,11 +647,9 @@ def run_c5(args, torch):
 
 
 def cpu_baseline_sweep(images, arrivals, window, top_k, args):
-    """The oracle's sweep on a centre crop of the first request with 8 sources (its planes), `cores`
-    concurrent copies: Mpixel-hyp/s = cores * crop pixels * planes / wall."""
+    """The oracle's sweep on a centre crop of the first request with 8 sources (its planes)."""
     import oracle
     from py_ortho_amd import block
-    cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
     q = next((q for _, q in arrivals if len(q.srcs) == 8), arrivals[0][1])
     img, cam = images[q.ref]
     H, W = img.shape
@@ -656,6 +659,18 @@ def cpu_baseline_sweep(images, arrivals, window, top_k, args):
     srcs = [images[k][0] for k in q.srcs]
     geo = np.stack([block._geo18(images[k][1], cam) for k in q.srcs])
     planes = block.inverse_depth_planes(q.dmin, q.dmax, q.planes)
+    return cpu_sweep_rate(ref, mu, sd, ray, srcs, geo, planes, window, top_k, args,
+                          f"a {W}x{H} request, S={len(q.srcs)}")
+
+
+def cpu_sweep_rate(ref, mu, sd, ray, srcs, geo, planes, window, top_k, args, what):
+    """The oracle's sweep (FP64, the reference's arithmetic) over all planes on a centre crop sized
+    to ~1.5 s per core, `cores` concurrent copies (one view per thread, densify.py:596):
+    Mpixel-hyp/s = cores * crop pixels * planes / wall."""
+    import oracle
+    cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
+    H, W = ref.shape
+    nplanes = len(planes)
 
     def sweep(side, nd, threads):
         y0, x0 = (H - side) // 2, (W - side) // 2
@@ -669,12 +684,163 @@ def cpu_baseline_sweep(images, arrivals, window, top_k, args):
         return time.perf_counter() - t
 
     per = sweep(16, 8, 1) / (256 * 8)
-    side = int(max(16, min(512, math.sqrt(1.5 / (per * q.planes)))))
-    wall = sweep(side, q.planes, cores)
-    return {"value": cores * side * side * q.planes / wall / 1e6, "unit": "Mpixel-hyp/s", "cores": cores,
-            "kind": "port", "sample": f"{cores} concurrent copies of the oracle sweep ({q.planes} planes, S="
-                                      f"{len(q.srcs)}, 11x11 window) on the {side}x{side} centre crop of a {W}x{H} "
-                                      f"request ({wall:.1f} s wall)"}
+    side = int(max(16, min(min(H, W), 512, math.sqrt(1.5 / (per * nplanes)))))
+    wall = sweep(side, nplanes, cores)
+    return {"value": cores * side * side * nplanes / wall / 1e6, "unit": "Mpixel-hyp/s", "cores": cores,
+            "kind": "port", "sample": f"{cores} concurrent copies of the oracle sweep ({nplanes} planes, "
+                                      f"{2 * window + 1}x{2 * window + 1} window) on the {side}x{side} centre crop of "
+                                      f"{what} ({wall:.1f} s wall)"}
+
+
+# ------------------------------------------------------------------------------- C4 precision variant
+def run_c4(args, torch):
+    """4000x3000 reference, S = 12, 384 planes + parabola + 8 PatchMatch iterations, with FP16 or BF16
+    image storage (--fmt); the same work in FP64 is run once beside it to state the error."""
+    from py_ortho_amd import _lib, batch, block, dense, scene
+    lib = _lib.lib()
+    t_scene = time.perf_counter()
+    sc = scene.config_c4(device="cuda", scale=args.scale)
+    torch.cuda.synchronize()
+    t_scene = time.perf_counter() - t_scene
+    ref = sc.ref.img.contiguous()
+    H, W = ref.shape
+    S = len(sc.srcs)
+    r, k, D, iters, pr, pstep = 5, 2, 384, 8, 5, 2
+    dmin, dmax = 73.0, 92.0
+    ray = scene.rays_world(sc.ref.cam, "cuda")
+    mu, sd = dense.ref_stats(ref, r)
+    geo = sc.geo()
+    hyp = {"mode": "const", "inv": torch.from_numpy(block.inverse_depth_planes(dmin, dmax, D)).cuda()}
+    srcs = [v.img.contiguous() for v in sc.srcs]
+    deps = [v.depth.contiguous() for v in sc.srcs]
+    active = torch.ones((H, W), dtype=torch.uint8, device="cuda")
+    fmt = args.fmt
+    ref16 = dense.to_lp(ref, fmt)
+    srcs16 = [dense.to_lp(x, fmt) for x in srcs]
+
+    def step(lp: bool):
+        st = dense.SweepState(H, W)
+        if lp:
+            dense.sweep_lp(ref16, mu, sd, srcs16, ray, geo, hyp, r, k, fmt, state=st)
+        else:
+            dense.sweep(ref, mu, sd, srcs, ray, geo, hyp, r, k, state=st)
+        d = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, r, k, st)
+        d_pm = d.clone()
+        nrm = batch.init_normals(d, ray)
+        score = st.best.clone()
+        dense.pm_refine(ref, ray, sc.ref.cam.C, srcs16 if lp else srcs, deps, sc.par(), d_pm, nrm, score, active,
+                        pr, pstep, k, 0.5, 0.01, iters, dmin, dmax, 2654435761, fmt=fmt if lp else None)
+        return st, d, d_pm
+
+    for _ in range(args.warmup):
+        step(True)
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    prof_read(lib, "sweep_lp")
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        st_lp, d_lp, pm_lp = step(True)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    lib.om_profile_enable(0)
+    sw_ms, sw_n = prof_read(lib, "sweep_lp")
+    pr_ms, pr_n = prof_read(lib, "pm_propagate")
+    lib.om_profile_read(None, None, None)
+    ms = wall / args.steps * 1e3
+    torch.cuda.synchronize()
+    t64 = time.perf_counter()
+    st64, d64, pm64 = step(False)
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t64) * 1e3
+    valid = (st64.best > 0.3) & (64.0 * sd > 0.5)
+    same = float((st_lp.idx == st64.idx)[valid].float().mean())
+    db = (st_lp.best - st64.best).abs()[valid]
+    rel_sw = ((d_lp - d64).abs() / d64)[valid]
+    both = torch.isfinite(pm_lp) & torch.isfinite(pm64)
+    rel_pm = ((pm_lp - pm64).abs() / pm64)[both]
+    truth = sc.ref.depth
+    px_h = H * W * (D + 11 * iters)
+    flops = H * W * D * sweep_flops_per_hyp(S)
+    sw_avg = sw_ms / max(sw_n, 1) / 1e3
+    res = {
+        "metric": METRIC, "value": px_h / (ms / 1e3) / 1e6, "unit": "Mpixel-hyp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": f"{fmt} storage, f32 arithmetic (f64 plane and variance guard)",
+        "data": "synthetic",
+        "config": {"workload": f"C4 precision variant: {W}x{H} reference, S={S}, {D} planes 11x11 NCC top-2 + "
+                               f"parabola, then {iters} PatchMatch iterations (11x11 window step 2), {fmt} images",
+                   "H": H, "W": W, "sources": S, "planes": D, "pm_iterations": iters, "pixel_hypotheses": px_h,
+                   "scene_s": t_scene},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "fp64_same_work_ms": t64, "speedup_vs_fp64": t64 / ms,
+        "error_vs_fp64": {"sweep_same_winner": same, "sweep_abs_best_median": float(db.median()),
+                          "sweep_abs_best_p99": float(torch.quantile(db.float()[:: max(1, db.numel() // 1000000)],
+                                                                     0.99)),
+                          "sweep_depth_rel_p99": float(torch.quantile(rel_sw.float()[:: max(1, rel_sw.numel()
+                                                                                              // 1000000)], 0.99)),
+                          "pm_depth_within_0.2pct": float((rel_pm < 2e-3).float().mean())},
+        "accuracy_vs_truth": {"pm_median_rel_err": float((((pm_lp - truth).abs() / truth)[both]).median()),
+                              "pm_fp64_median_rel_err": float((((pm64 - truth).abs() / truth)[both]).median())},
+        "roofline": {"bound": "mfma", "engine": "VALU FP32 (no MFMA: gather-and-reduce)",
+                     "achieved": flops / sw_avg / 1e12 if sw_avg else 0.0, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": flops / sw_avg / 157.3e12 if sw_avg else 0.0, "traffic": None, "kernel": "sweep_lp_kernel",
+                     "kernel_avg_ms": sw_avg * 1e3, "pm_propagate_ms_per_step": pr_ms / max(args.steps, 1)},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_view(
+            ref.cpu().numpy(), mu.cpu().numpy(), sd.cpu().numpy(), ray.cpu().numpy(), [x.cpu().numpy() for x in srcs],
+            geo, sc.par(), np.asarray(sc.ref.cam.C, np.float64), [x.cpu().numpy() for x in deps], d64.cpu().numpy(),
+            batch.init_normals(d64, ray).cpu().numpy(), hyp["inv"].cpu().numpy(), r, k, pr, pstep, iters, dmin, dmax,
+            args)
+    return res
+
+
+def cpu_baseline_view(ref, mu, sd, ray, srcs, geo, par, Cr, deps, depth0, normal0, planes, r, k, pr, pstep, iters,
+                      dmin, dmax, args):
+    """The oracle (FP64, the reference's arithmetic) on centre crops of one view: the sweep over all
+    planes and pm_refine for all iterations, `cores` concurrent copies each; Mpixel-hyp/s = cores *
+    (planes + 11 iterations) / (sweep + PatchMatch wall time per crop pixel)."""
+    import oracle
+    cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
+    H, W = ref.shape
+    D = len(planes)
+
+    def crop(side):
+        y0, x0 = (H - side) // 2, (W - side) // 2
+        return y0, x0, np.s_[y0:y0 + side, x0:x0 + side]
+
+    def sweep(side, nd, threads):
+        y0, x0, sl = crop(side)
+        base = np.full((side, side), planes[0], np.float32)
+        step = np.full((side, side), planes[1] - planes[0], np.float32)
+        t = time.perf_counter()
+        oracle.bench_sweep(threads, ref[sl], mu[sl], sd[sl], srcs, np.ascontiguousarray(ray[:, y0:y0 + side,
+                                                                                         x0:x0 + side]),
+                           geo, base, step, nd, r, k)
+        return time.perf_counter() - t
+
+    def pm(side, it, threads):
+        y0, x0, sl = crop(side)
+        t = time.perf_counter()
+        oracle.bench_pm(threads, ref[sl], np.ascontiguousarray(ray[:, y0:y0 + side, x0:x0 + side]), Cr, srcs, deps,
+                        par, np.ascontiguousarray(depth0[sl]), np.ascontiguousarray(normal0[:, y0:y0 + side,
+                                                                                          x0:x0 + side]),
+                        np.ones((side, side), np.uint8), pr, pstep, k, 0.5, 0.01, it, dmin, dmax, 1)
+        return time.perf_counter() - t
+
+    target = 1.5
+    side_s = int(max(16, min(512, math.sqrt(target / (sweep(16, 8, 1) / (256 * 8) * D)))))
+    side_p = int(max(16, min(512, math.sqrt(target / (pm(16, 1, 1) / 256 * iters)))))
+    ws = sweep(side_s, D, cores)
+    wp = pm(side_p, iters, cores)
+    per_px = ws / (side_s * side_s) + wp / (side_p * side_p)
+    return {"value": cores * (D + 11 * iters) / per_px / 1e6, "unit": "Mpixel-hyp/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent copies of the oracle sweep ({D} planes, {side_s}x{side_s} centre crop) and "
+                      f"pm_refine ({iters} iterations, {side_p}x{side_p} crop), S={len(srcs)}; "
+                      f"{ws:.1f} s + {wp:.1f} s wall"}
 
 
 def main(argv=None):
@@ -682,12 +848,15 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--fmt", choices=["f16", "bf16"], default="f16", help="C4: image storage format")
     ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
     ap.add_argument("--duration", type=float, default=3.0, help="C5: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="C5: admission budget (GB of HBM)")
     ap.add_argument("--step-planes", type=int, default=32, help="C5: planes per request per step")
     ap.add_argument("--plain-requests", type=int, default=24, help="C5: requests timed one at a time")
+    ap.add_argument("--active-mpix", type=float, default=48.0,
+                    help="C5: megapixels of admitted requests in flight (latency under overload)")
     ap.add_argument("--scale", type=float, default=1.0, help="C3: image size scale (1 = the full block)")
     ap.add_argument("--plain-per-hyp", action="store_true",
                     help="C3: also time the per-view form with one sweep launch per plane")
@@ -713,7 +882,7 @@ def main(argv=None):
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c5": run_c5}[args.workload](args, torch)
+    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -337,3 +337,19 @@ def config_c5_pools(device="cpu", scale: float = 1.0, nx: int = 4, ny: int = 4) 
         out.append(make_survey(6 + c, cams, counts, ext, 90.0, device=device, n_bumps=16, march=64, bisect=30))
     return out
 
+
+def config_c4(device="cpu", scale: float = 1.0) -> Scene:
+    """Large view at reduced precision (seed 5): a 4000x3000 reference (f = 0.9375 W, 90 m) at the
+    centre of a 5 x 3 nadir grid (70/60 % overlap) with its 12 nearest views as sources."""
+    W, H = int(round(4000 * scale)), int(round(3000 * scale))
+    cams, counts, ext = survey_cameras(5, 5, 3, 90.0, [(W, H)], 0.9375, 0.7, 0.6, (12, 12))
+    C = np.stack([c.C for c in cams])
+    k = 7                                           # the centre of the 5 x 3 grid
+    nb = neighbours_by_distance(C, 90.0, [12] * len(cams))[k]
+    terr = make_terrain(5, ext, n_bumps=16)
+    views = []
+    for j in [k] + nb:
+        img, dep = render(terr, cams[j], noise_seed=5000 + j, device=device, march=64, bisect=30)
+        views.append(View(cams[j], img, dep))
+    return Scene(terr, views[0], views[1:])
+

@@ -332,12 +332,16 @@ class DepthServer:
     requests are cancelled)."""
 
     def __init__(self, store: ImageStore, executor, budget_bytes: float = 200e9, step_planes: int = 32,
-                 lookahead: int = 2, max_active: int = 256, clock: Callable[[], float] = time.perf_counter):
+                 lookahead: int = 2, max_active: int = 256, max_active_pixels: float = float("inf"),
+                 clock: Callable[[], float] = time.perf_counter):
         self.store, self.ex = store, executor
         self.adm = Admission(budget_bytes)
         self.step_planes = int(step_planes)
         self.lookahead = int(lookahead)
         self.max_active = int(max_active)
+        # reference pixels in flight: enough to keep the device busy, few enough that admitted
+        # requests finish quickly instead of all advancing together (latency under overload)
+        self.max_active_pixels = float(max_active_pixels)
         self.clock = clock
         self.active: List[_Active] = []
         self._inflight: "collections.deque" = collections.deque()   # (event, finished list)
@@ -398,12 +402,25 @@ class DepthServer:
             return not self.active and not self.adm.queue and not self._inflight
 
     # ---- the loop
+    def _pixels(self, a: _Active) -> int:
+        return int(np.prod(np.shape(self.store.loader(a.req.ref)[0])))
+
     def _admit(self):
         with self._cv:
             room = self.max_active - len(self.active)
             if room <= 0:
                 return
-            got = self.adm.take(lambda a: a.need, lambda a: self.store.pinned_bytes_with(a.keys), room)
+            px = sum(self._pixels(a) for a in self.active)
+            got = []
+            while self.adm.queue and len(got) < room:
+                a = self.adm.queue[0]
+                p = self._pixels(a)
+                if (self.active or got) and px + p > self.max_active_pixels:
+                    break
+                if not self.adm.take(lambda x: x.need, lambda x: self.store.pinned_bytes_with(x.keys), 1):
+                    break
+                px += p
+                got.append(a)
         for a in got:
             try:
                 a.t_admit = self.clock()

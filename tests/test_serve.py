@@ -89,7 +89,7 @@ def _server(sizes, budget, step_planes=16, **kw):
     store = ImageStore(_loader(sizes), budget)
     ex = FakeExecutor(ref, **{k: v for k, v in kw.items() if k in ("fail_on", "step_delay")})
     srv = DepthServer(store, ex, budget_bytes=budget, step_planes=step_planes,
-                      **{k: v for k, v in kw.items() if k in ("max_active", "lookahead")})
+                      **{k: v for k, v in kw.items() if k in ("max_active", "lookahead", "max_active_pixels")})
     ref[0] = srv
     return srv, store, ex
 
@@ -215,3 +215,18 @@ def test_stop_without_drain_cancels_queue():
     late = srv.submit(DepthRequest("v0", ["v1"], 10.0, 20.0, 8))
     with pytest.raises(RuntimeError):
         late.result(timeout=1)
+
+
+def test_pixel_limit_bounds_the_step_but_never_starves():
+    sizes = {f"v{i}": (30, 40) for i in range(8)}     # 1200 px each
+    sizes["big"] = (100, 100)                         # alone above the limit: still admitted when idle
+    srv, store, ex = _server(sizes, 10 ** 9, step_planes=8, max_active_pixels=2500)
+    futs = [srv.submit(DepthRequest(f"v{i}", [f"v{(i + 1) % 8}"], 10.0, 20.0, 32)) for i in range(6)]
+    futs.append(srv.submit(DepthRequest("big", ["v0"], 10.0, 20.0, 16)))
+    srv.start()
+    for f in futs:
+        f.result(timeout=10)
+    srv.stop()
+    assert max(len(s) for s in ex.steps) <= 2
+    assert srv.stats["finished"] == 7
+
