@@ -275,7 +275,7 @@ def run_c1(args, torch):
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
         "roofline": {"bound": "mfma", "engine": "VALU FP64", "achieved": flops / avg / 1e12 if avg else 0,
                      "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": (flops / avg / FP64_PEAK) if avg else 0,
-                     "traffic": None, "kernel": "sweep_kernel", "kernel_avg_ms": avg * 1e3},
+                     "traffic": profiled_traffic("sweep_c1.md"), "kernel": "sweep_kernel", "kernel_avg_ms": avg * 1e3},
         "accuracy": {"valid_frac": float(valid.float().mean()),
                      "valid_within_1pct": float((rel[valid] < 0.01).float().mean())},
     }
