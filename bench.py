@@ -13,7 +13,7 @@ and iteration: current + 8 propagated + 2 random).  Also printed: p50/p99 step l
 dominant kernel's roofline (HIP events around its launches), and the oracle (C restatement of the
 reference) timed on this host's cores on a bounded centre crop of the same workload.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c2|c1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c2|c1|c3|c4|c5] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -122,19 +122,22 @@ def run_c2(args, torch):
     normal = torch.empty_like(c["normal0"])
     score = torch.empty((H, W), dtype=torch.float32, device="cuda")
 
+    stats = {}
+
     def step():
         depth.copy_(c["depth0"])
         normal.copy_(c["normal0"])
         score.fill_(-1.0)
         return dense.pm_refine(c["ref"], c["ray"], c["Cr"], c["srcs"], c["deps"], c["par"], depth, normal, score,
                                c["active"], c["r"], c["step"], c["top_k"], c["geo_w"], c["geo_tol"], c["iters"],
-                               c["dmin"], c["dmax"], c["seed"])
+                               c["dmin"], c["dmax"], c["seed"], stats=stats)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     lib.om_profile_enable(1)
-    prof_read(lib, "pm_propagate")  # drop warm-up records
+    for name in ("pm_propagate", "pm_eval", "pm_random"):
+        prof_read(lib, name)  # drop warm-up records
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -147,6 +150,7 @@ def run_c2(args, torch):
     wall = time.perf_counter() - t0
     lib.om_profile_enable(0)
     prop_ms, prop_n = prof_read(lib, "pm_propagate")
+    eval_ms, eval_n = prof_read(lib, "pm_eval")
     rand_ms, rand_n = prof_read(lib, "pm_random")
     lib.om_profile_read(None, None, None)
 
@@ -155,9 +159,18 @@ def run_c2(args, torch):
     nsamp = ((2 * c["r"]) // c["step"] + 1) ** 2
     fpe = pm_flops_per_eval(S, nsamp)
     # dominant kernel: propagation, 9 nominal evaluations per pixel of one colour
+    # The propagation pass is memoised: pm_need lists the candidate costs whose plane changed since
+    # the pixel's last visit, pm_eval_kernel computes them (the dominant kernel), pm_decide replays
+    # the accept sequence.  Algorithmic FLOPs per pm_eval_kernel launch = the evaluations it performed
+    # (counted on the device) x FLOPs per evaluation; the reference's nominal work per pass is
+    # (H W / 2) x 9 evaluations.
+    passes = 2 * c["iters"]
+    evaluated = stats.get("evaluated", -1)
+    nominal_evals = (H * W / 2) * 9 * passes
+    eval_avg = eval_ms / max(eval_n, 1) / 1e3
     prop_avg = prop_ms / max(prop_n, 1) / 1e3
-    prop_flops = (H * W / 2) * 9 * fpe
-    achieved = prop_flops / prop_avg if prop_avg > 0 else 0.0
+    prop_flops = (evaluated / passes if evaluated >= 0 else (H * W / 2) * 9) * fpe
+    achieved = prop_flops / eval_avg if eval_avg > 0 else 0.0
     # accuracy of the refined map against the rendered truth
     truth = c["scene"].ref.depth
     rel = ((depth - truth).abs() / truth).float()
@@ -174,8 +187,12 @@ def run_c2(args, torch):
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
         "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)", "achieved": achieved / 1e12,
                      "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
-                     "traffic": profiled_traffic("pm_propagate_c2.md"), "kernel": "pm_propagate_kernel", "kernel_avg_ms": prop_avg * 1e3, "launches": prop_n,
+                     "traffic": profiled_traffic("pm_eval_c2.md"), "kernel": "pm_eval_kernel",
+                     "kernel_avg_ms": eval_avg * 1e3, "launches": eval_n,
                      "flops_per_launch": prop_flops, "flops_per_eval": fpe,
+                     "evaluated_per_step": evaluated, "nominal_evaluations_per_step": nominal_evals,
+                     "propagation_pass_avg_ms": prop_avg * 1e3,
+                     "nominal_tflops": (H * W / 2) * 9 * fpe / prop_avg / 1e12 if prop_avg else 0.0,
                      "pm_random_avg_ms": rand_ms / max(rand_n, 1)},
         "accuracy": acc,
     }

@@ -207,6 +207,11 @@ int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream);
  * the tests and benchmark. */
 int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref16, void *stream);
 
+/* After om_pm_refine / om_pm_refine_lp on `workspace`: the propagation cost evaluations actually
+ * computed (the memoised path reuses a candidate's cost while its plane is unchanged; the reference
+ * evaluates H*W*9 per iteration, _dense.pyx:614-645); -1 when the call had no memo (very large views). */
+int om_pm_workspace_counts(const void *workspace, int32_t H, int32_t W, int64_t *evaluated);
+
 /* The precision variant of om_pm_refine (no reference counterpart: the reference's cost is FP64,
  * _dense.pyx:461-556): source images imgs16[s] stored as FP16 or BF16 (fmt), the plane (num = d n.ray)
  * and the geometric term in FP64, per sample the ray-plane intersection, projection and bilinear tap

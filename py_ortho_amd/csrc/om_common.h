@@ -36,7 +36,7 @@ inline int launch_check(const char *what) { return hip_check(hipGetLastError(), 
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
-inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with its own L2.
 // xcd_block maps the hardware block index to a logical one so that every XCD walks a contiguous

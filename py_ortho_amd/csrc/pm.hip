@@ -102,6 +102,13 @@ struct PMK {
     unsigned long long *changed;
     const struct PMCam *cam;   // (S,) per-source tables in device memory
     const float *depth_from, *score_from;  // batched runs: start maps copied in first (or NULL)
+    // cost memo of the single-view path: per pixel the (cost, photometric score) of its 8 neighbour
+    // candidates and its own plane, whether its plane changed at its last visit, and the list of
+    // (pixel, slot) evaluations a pass needs
+    double *cc;
+    uint8_t *chg;
+    int *items, *nitems;
+    unsigned long long *nevals;  // cost evaluations the memo path performed (all passes of a call)
 };
 
 // One source view as the cost kernels read it (device memory, uniform loads): the reference's
@@ -447,6 +454,7 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
 struct PassK {
     int colour, parity;
     double dsc, nsc;  // 0.05 / (it+1), 0.3 / (it+1) (_dense.pyx:609-610)
+    int first;        // first iteration: no memoised costs yet
 };
 
 __device__ __forceinline__ bool colour_pixel(const PMK &P, const PassK &Q, int64_t t, int &y, int &x)
@@ -626,6 +634,184 @@ __device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64
         }
     }
     P.score[p] = (float)sc;
+    if (P.chg && changed)
+        P.chg[p] = 1;
+    count_changes(P, changed);
+}
+
+// ---------------------------------------------------------------- memoised propagation
+// The cost of a plane at a pixel depends only on the pixel, the plane and inputs that a call never
+// changes, so a candidate whose plane is the same as at the pixel's previous visit has the same
+// cost.  A neighbour's candidate plane is a function of the neighbour's depth and normal only, and
+// those change only at the neighbour's own visit; the pixel's own plane changes only at its own.
+// Per colour pass: (1) list the (pixel, slot) costs that must be computed -- slot 8 is the pixel's
+// own plane, slots 0..7 its neighbours' -- (2) compute them, one lane per item, (3) replay the
+// reference's accept sequence (_dense.pyx:614-645) from the memo.  Bit-identical to evaluating
+// every candidate, with dense lanes however few candidates changed.
+__device__ const int kOffY[8] = {-1, 1, 0, 0, -5, 5, 0, 0};
+__device__ const int kOffX[8] = {0, 0, -1, 1, 0, 0, -5, 5};
+
+// the candidate plane that neighbour k proposes to pixel p (_dense.pyx:621-635); false if none
+__device__ __forceinline__ bool pm_candidate(const PMK &P, int y, int x, int k, double &dp, double &b0, double &b1,
+                                             double &b2)
+{
+    const int yn = y + kOffY[k], xn = x + kOffX[k];
+    if (yn < 0 || xn < 0 || yn >= P.H || xn >= P.W)
+        return false;
+    const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x, q = (size_t)yn * P.W + xn;
+    const float dq = P.depth[q];
+    if (!isfinite(dq))
+        return false;
+    b0 = P.normal[q];
+    b1 = P.normal[HW + q];
+    b2 = P.normal[2 * HW + q];
+    const double dn = b0 * P.ray[q] + b1 * P.ray[HW + q] + b2 * P.ray[2 * HW + q];
+    const double ndr = b0 * P.ray[p] + b1 * P.ray[HW + p] + b2 * P.ray[2 * HW + p];
+    if (ndr >= -1e-6)
+        return false;
+    dp = (double)dq * dn / ndr;
+    return !(dp < P.dmin || dp > P.dmax);
+}
+
+__global__ void pm_memo_reset_kernel(const PMK P, int64_t HW, int clear_chg)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *P.nitems = 0;
+        if (clear_chg)
+            *P.nevals = 0ull;
+    }
+    if (clear_chg && i < HW)
+        P.chg[i] = 0;
+}
+
+// (1) which costs the pass needs, appended to the item list a wave at a time
+__global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const PassK Q)
+{
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int y, x;
+    unsigned mask = 0;
+    size_t p = 0;
+    if (colour_pixel(P, Q, t, y, x)) {
+        p = (size_t)y * P.W + x;
+        if (P.active[p]) {
+            if (Q.first || P.chg[p])
+                mask |= 1u << 8;
+            for (int k = 0; k < 8; ++k) {
+                const int yn = y + kOffY[k], xn = x + kOffX[k];
+                if (yn < 0 || xn < 0 || yn >= P.H || xn >= P.W)
+                    continue;
+                if (Q.first || P.chg[(size_t)yn * P.W + xn])
+                    mask |= 1u << k;
+            }
+        }
+    }
+    // slot-major order within the wave: the items of one slot over consecutive pixels sit together,
+    // so that a wave of the evaluation kernel mostly runs one neighbour direction in lockstep
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int base = 0;
+    {
+        int total = 0;
+        for (int slot = 0; slot < 9; ++slot)
+            total += __popcll(__ballot((mask >> slot) & 1u));
+        if (lane == 0 && total) {
+            base = atomicAdd(P.nitems, total);
+            atomicAdd(P.nevals, (unsigned long long)total);
+        }
+        base = __shfl(base, 0, 64);
+    }
+    for (int slot = 0; slot < 9; ++slot) {
+        const bool on = (mask >> slot) & 1u;
+        const unsigned long long m = __ballot(on);
+        if (on)
+            P.items[base + __popcll(m & below)] = (int)(p * 16 + slot);
+        base += __popcll(m);
+    }
+}
+
+// (2) one cost per lane.  The grid covers the largest possible list, the item count is known only
+// on the device: the blocks that have items are mapped so that each XCD (hardware dispatch is
+// round-robin over the 8 XCDs) gets an equal, contiguous share -- equal work, and neighbouring
+// items in one L2 -- and the surplus blocks of every XCD exit at once.
+template <int SM, int FMT>
+__device__ __forceinline__ void pm_eval_item(const PMK &P, int64_t i);
+
+template <int SM, int FMT>
+__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_eval_kernel(const PMK P)
+{
+    const int64_t n = *P.nitems;
+    const int64_t used = cdiv(n, (int64_t)blockDim.x);           // blocks with items
+    const int64_t per = cdiv(used, (int64_t)kXCD);                // of them per XCD
+    const int xcd = blockIdx.x % kXCD, j = blockIdx.x / kXCD;
+    if (j >= per)
+        return;
+    const int64_t i = ((int64_t)xcd * per + j) * blockDim.x + threadIdx.x;
+    if (i < n)
+        pm_eval_item<SM, FMT>(P, i);
+}
+
+template <int SM, int FMT>
+__device__ __forceinline__ void pm_eval_item(const PMK &P, int64_t i)
+{
+    const int it = P.items[i];
+    const size_t p = (size_t)(it >> 4);
+    const int slot = it & 15;
+    const int y = (int)(p / P.W), x = (int)(p - (size_t)y * P.W);
+    const size_t HW = (size_t)P.H * P.W;
+    // the plane of the slot (the pixel's own, or neighbour slot's candidate), then one cost call
+    double dp = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    bool ok;
+    if (slot == 8) {
+        const float d0 = P.depth[p];
+        ok = isfinite(d0);
+        dp = d0;
+        b0 = P.normal[p];
+        b1 = P.normal[HW + p];
+        b2 = P.normal[2 * HW + p];
+    } else {
+        ok = pm_candidate(P, y, x, slot, dp, b0, b1, b2);
+    }
+    double cost = slot == 8 ? 1e9 : NAN, photo = -1.0;  // no plane: the reference's 1e9 / -1, or skip
+    if (ok)
+        cost = pm_cost<SM, FMT>(P, y, x, dp, b0, b1, b2, photo);
+    P.cc[(p * 9 + slot) * 2] = cost;
+    P.cc[(p * 9 + slot) * 2 + 1] = photo;
+}
+
+// (3) the reference's accept sequence over the memoised costs
+__global__ void __launch_bounds__(kPMThreads) pm_decide_kernel(const PMK P, const PassK Q)
+{
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int y, x;
+    if (!colour_pixel(P, Q, t, y, x))
+        return;
+    const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
+    if (!P.active[p])
+        return;
+    const double *c = P.cc + p * 18;
+    double best = c[16], sc = c[17];
+    int changed = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int yn = y + kOffY[k], xn = x + kOffX[k];
+        if (yn < 0 || xn < 0 || yn >= P.H || xn >= P.W)  // never listed, so never written
+            continue;
+        const double cst = c[2 * k];
+        if (!(cst < best))  // NaN: no candidate from this neighbour
+            continue;
+        double dp, b0, b1, b2;
+        pm_candidate(P, y, x, k, dp, b0, b1, b2);
+        best = cst;
+        sc = c[2 * k + 1];
+        P.depth[p] = (float)dp;
+        P.normal[p] = (float)b0;
+        P.normal[HW + p] = (float)b1;
+        P.normal[2 * HW + p] = (float)b2;
+        changed += 1;
+    }
+    P.best[p] = best;
+    P.sc[p] = sc;
+    P.chg[p] = changed ? 1 : 0;
     count_changes(P, changed);
 }
 
@@ -716,11 +902,18 @@ struct PMWs {
     uint32_t *st;  // 2 states
     unsigned long long *changed;
     PMCam *cam;    // 16 source tables
+    double *cc;    // memo (single view): (H, W, 9, 2)
+    uint8_t *chg;  // (H, W)
+    int *items, *nitems;
+    unsigned long long *nevals;
 };
+
+// the memo path lists items as pixel * 16 + slot in an int
+static bool pm_memo_ok(int H, int W) { return (int64_t)H * W < (1LL << 27); }
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-static size_t pm_ws_layout(int H, int W, char *base, PMWs *w)
+static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
 {
     const size_t HW = (size_t)H * W;
     size_t o = 0;
@@ -731,7 +924,21 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w)
     };
     char *b = take(HW * 8), *s = take(HW * 8), *rk = take(HW * 4), *rc = take((size_t)H * 4),
          *ro = take((size_t)H * 4), *st = take(16), *ch = take(8), *cm = take(sizeof(PMCam) * 16);
+    char *cc = nullptr, *cg = nullptr, *it = nullptr, *ni = nullptr, *ne = nullptr;
+    if (memo) {
+        const size_t ncol = (size_t)H * ((W + 1) / 2);
+        cc = take(HW * 18 * sizeof(double));
+        cg = take(HW);
+        it = take(ncol * 9 * sizeof(int));
+        ni = take(sizeof(int));
+        ne = take(sizeof(unsigned long long));
+    }
     if (w) {
+        w->cc = (double *)cc;
+        w->chg = (uint8_t *)cg;
+        w->items = (int *)it;
+        w->nitems = (int *)ni;
+        w->nevals = (unsigned long long *)ne;
         w->best = (double *)b;
         w->sc = (double *)s;
         w->rank = (int *)rk;
@@ -751,7 +958,18 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
     const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
     P.cam = w.cam;
     P.st = w.st;
+    const bool memo = w.cc != nullptr;
+    P.cc = w.cc;
+    P.chg = w.chg;
+    P.items = w.items;
+    P.nitems = w.nitems;
+    P.nevals = w.nevals;
+    const int64_t HW = (int64_t)P.H * P.W;
+    // one block per 128 possible items, rounded up to whole rounds over the XCDs
+    const int64_t eval_grid = cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD;
     hipLaunchKernelGGL(pm_init_kernel, dim3(1), dim3(1), 0, stream, w.st, seed, w.changed, A, w.cam);
+    if (memo)
+        hipLaunchKernelGGL(pm_memo_reset_kernel, dim3((unsigned)cdiv(HW, 256)), dim3(256), 0, stream, P, HW, 1);
     int pass = 0;
     for (int it = 0; it < iters; ++it) {
         for (int colour = 0; colour < 2; ++colour, ++pass) {
@@ -760,7 +978,19 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
             Q.parity = pass & 1;
             Q.dsc = 0.05 / (it + 1);
             Q.nsc = 0.3 / (it + 1);
-            {
+            Q.first = it == 0;
+            if (memo) {
+                ProfScope ps("pm_propagate", stream);
+                if (pass > 0)
+                    hipLaunchKernelGGL(pm_memo_reset_kernel, dim3(1), dim3(64), 0, stream, P, HW, 0);
+                hipLaunchKernelGGL(pm_need_kernel, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                {
+                    ProfScope pe("pm_eval", stream);
+                    hipLaunchKernelGGL((pm_eval_kernel<SM, FMT>), dim3((unsigned)eval_grid), dim3(kPMThreads), 0,
+                                       stream, P);
+                }
+                hipLaunchKernelGGL(pm_decide_kernel, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+            } else {
                 ProfScope ps("pm_propagate", stream);
                 hipLaunchKernelGGL((pm_propagate_kernel<SM, FMT>), dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
             }
@@ -789,7 +1019,7 @@ extern "C" size_t om_pm_workspace_bytes(int32_t H, int32_t W)
 {
     if (H <= 0 || W <= 0)
         return 256;
-    return pm_ws_layout(H, W, nullptr, nullptr);
+    return pm_ws_layout(H, W, nullptr, nullptr, pm_memo_ok(H, W));
 }
 
 static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
@@ -813,11 +1043,12 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
         return fail(OM_EINVAL, "null host array");
     if ((int64_t)H * W == 0 || iters <= 0)
         return OM_OK;
-    const size_t need = pm_ws_layout(H, W, nullptr, nullptr);
+    const bool memo = pm_memo_ok(H, W);
+    const size_t need = pm_ws_layout(H, W, nullptr, nullptr, memo);
     if (!workspace || workspace_bytes < need)
         return fail(OM_EINVAL, "workspace of %zu bytes needed (got %zu)", need, workspace_bytes);
     PMWs w;
-    pm_ws_layout(H, W, (char *)workspace, &w);
+    pm_ws_layout(H, W, (char *)workspace, &w, memo);
     PMK P;
     memset(&P, 0, sizeof P);
     P.ref = ref;
@@ -897,6 +1128,22 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
         *changed = (int64_t)h;
     }
     return OM_OK;
+}
+
+extern "C" int om_pm_workspace_counts(const void *workspace, int32_t H, int32_t W, int64_t *evaluated)
+{
+    if (!workspace || !evaluated)
+        return fail(OM_EINVAL, "null argument");
+    *evaluated = -1;
+    if (H <= 0 || W <= 0 || !pm_memo_ok(H, W))
+        return OM_OK;
+    PMWs w;
+    pm_ws_layout(H, W, (char *)workspace, &w, true);
+    unsigned long long h = 0;
+    int rc = hip_check(hipMemcpy(&h, w.nevals, sizeof h, hipMemcpyDeviceToHost), "copy counts");
+    if (!rc)
+        *evaluated = (int64_t)h;
+    return rc;
 }
 
 extern "C" int om_pm_refine(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
@@ -1183,6 +1430,7 @@ static void pm_batch_passes(om_batch *b, hipStream_t st)
             Q.parity = pass & 1;
             Q.dsc = 0.05 / (it + 1);
             Q.nsc = 0.3 / (it + 1);
+            Q.first = it == 0;
             {
                 ProfScope ps("pm_propagate_batch", st);
                 for (int gi = 0; gi < b->pm_ngroups; ++gi)

@@ -278,10 +278,11 @@ def rays_world(H: int, W: int, R, cx: float, cy: float, f: float, k1: float = 0.
 
 
 def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active, r, step, top_k, geo_w, geo_tol,
-              iters, dmin, dmax, seed, fmt: Optional[str] = None) -> int:
+              iters, dmin, dmax, seed, fmt: Optional[str] = None, stats: Optional[dict] = None) -> int:
     """PatchMatch repair of a depth map (_dense.pyx:566-666): only `active` pixels are re-estimated;
     depth / normal / score are updated in place.  Returns the number of accepted changes.
-    fmt "f16" / "bf16": the precision variant, with `srcs` as 16-bit patterns (to_lp)."""
+    fmt "f16" / "bf16": the precision variant, with `srcs` as 16-bit patterns (to_lp).
+    stats: if given, receives "evaluated", the propagation cost evaluations actually computed."""
     S = len(srcs)
     if S == 0 or S > 16:
         return 0
@@ -319,6 +320,10 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
     else:
         rc = L.om_pm_refine_lp(*args, FORMATS[fmt], _stream())
     check(rc, "pm_refine")
+    if stats is not None:
+        ev = ctypes.c_int64(0)
+        check(L.om_pm_workspace_counts(ws.data_ptr(), H, W, ctypes.byref(ev)), "pm counts")
+        stats["evaluated"] = int(ev.value)
     return int(changed.value)
 
 
