@@ -17,6 +17,8 @@
  *   om_hyp_refine         _fast.hyp_refine                  orthomosaic/_fast.pyx:644-658
  *   om_half               densify._half                     orthomosaic/densify.py:127-130
  *   om_ref_stats          mu / sd of _sweep                 orthomosaic/densify.py:327-328
+ *   om_consistency        densify._consistency              orthomosaic/densify.py:441-478
+ *   om_fuse               densify._fuse / _fuse_loop        orthomosaic/densify.py:676-762
  */
 #ifndef OM355_H
 #define OM355_H
@@ -223,6 +225,50 @@ int om_pm_refine_lp(const float *ref, int32_t H, int32_t W, const float *ray, co
                     int32_t step, int32_t top_k, double geo_w, double geo_tol, int32_t iters, double dmin,
                     double dmax, uint32_t seed, void *workspace, size_t workspace_bytes, int64_t *changed,
                     int32_t fmt, void *stream);
+
+/* Camera of densify._Cam (densify.py:133-160) with float64 parameters: R row-major world->camera,
+ * C the centre, pinhole f / (cx, cy) with radial k1, k2, k3, image w x h. */
+typedef struct om_cam {
+    double R[9];
+    double C[3];
+    double f, k1, k2, k3, cx, cy;
+    int32_t w, h;
+} om_cam;
+
+/* densify._consistency (densify.py:441-478): cnt (cam.h, cam.w) int16 = for each finite pixel of
+ * depth, the number of the n_nb (<= 16) neighbours that agree with it (forward-backward reprojection
+ * within px_tol pixels and relative depth within rel_tol[s]); 0 where depth is not finite.
+ * nb_depth[s]: device (nb_cam[s].h, nb_cam[s].w) float32 depth maps; nb_cam, rel_tol: host arrays.
+ * FP64 in the reference's order (_fast.pyx:201-274), 3x3 products as NumPy/OpenBLAS form them. */
+int om_consistency(const float *depth, const om_cam *cam, int32_t n_nb, const float *const *nb_depth,
+                   const om_cam *nb_cam, const double *rel_tol, double px_tol, int16_t *cnt, void *stream);
+
+/* One view of om_fuse: device depth (h, w) float32 (NaN = none), colours (h, w, 3) uint8 (may be
+ * NULL when cols is NULL), and its used flags on the stride grid (ceil(h / stride), ceil(w / stride))
+ * uint8, zero before the first call (the reference's used map, densify.py:657, is only set and read
+ * on grid pixels). */
+typedef struct om_fuse_view {
+    const float *depth;
+    const uint8_t *rgb;
+    uint8_t *used;
+    om_cam cam;
+} om_fuse_view;
+
+/* workspace of om_fuse: per-grid-pixel staging for the largest view plus block counts */
+size_t om_fuse_workspace_bytes(const om_fuse_view *views, int32_t n_views, int32_t stride);
+
+/* densify._fuse_loop (densify.py:688-762): sequential fusion of the views order[0..n_order) (host
+ * array).  nbrs (n_views, max_nb) host int32 neighbour lists, -1 for none (list a view only if it has
+ * a depth map, as `j in have` does); rel_tol (n_views, max_nb) host float64.  Appends the fused
+ * points in the reference's order: pts (cap, 3) float64, cols (cap, 3) uint8 (NULL to skip),
+ * nv (cap,) uint8 = min(views, 255), owner (cap,) int32 = the reference view.  *total (device
+ * int64) = the number of points when the stream reaches the end of the call; only the first cap
+ * are written, so total > cap means the output was too small.  Updates the views' used maps.
+ * Two launches per view on `stream`; the host arrays are read before the call returns. */
+int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, const om_fuse_view *views, const int32_t *nbrs,
+            int32_t max_nb, const double *rel_tol, double px_tol, int32_t min_views, int32_t stride, double *pts,
+            uint8_t *cols, uint8_t *nv, int32_t *owner, int64_t cap, int64_t *total, void *workspace,
+            size_t workspace_bytes, void *stream);
 
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);

@@ -329,3 +329,73 @@ def sample_view(img, cam, X0, Y0, gsd, Z):
                               _ptr(zd, _f32p), ctypes.c_int64(H), ctypes.c_int64(W), _ptr(od, _f32p), _ptr(vd, _u8p))
         os_[d], vs[d] = od, vd
     return out, valid
+
+
+# ------------------------------------------------------------------ consistency + fusion (om_fuse.c)
+class OrcCam(ctypes.Structure):
+    _fields_ = [("R", ctypes.c_double * 9), ("C", ctypes.c_double * 3), ("f", ctypes.c_double),
+                ("k1", ctypes.c_double), ("k2", ctypes.c_double), ("k3", ctypes.c_double),
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
+
+
+def orc_cam(cam) -> OrcCam:
+    """OrcCam of any object with R, C, f, k1, k2, k3, cx, cy, w, h (densify._Cam's fields)."""
+    c = OrcCam()
+    c.R[:] = [float(v) for v in np.asarray(cam.R, np.float64).ravel()]
+    c.C[:] = [float(v) for v in np.asarray(cam.C, np.float64).ravel()]
+    for k in ("f", "k1", "k2", "k3", "cx", "cy"):
+        setattr(c, k, float(getattr(cam, k)))
+    c.w, c.h = int(cam.w), int(cam.h)
+    return c
+
+
+def consistency(D, cam, nb_depths, nb_cams, rel_tol, px_tol):
+    """Restatement of densify._consistency (densify.py:441-478) -> int16 (H, W) counts."""
+    D = _c32(D)
+    cr = orc_cam(cam)
+    nd = [_c32(x) for x in nb_depths]
+    cams = (OrcCam * max(1, len(nd)))(*[orc_cam(c) for c in nb_cams])
+    ptrs = (ctypes.c_void_p * max(1, len(nd)))(*[x.ctypes.data for x in nd])
+    tol = np.ascontiguousarray(rel_tol, np.float64)
+    cnt = np.zeros(D.shape, np.int16)
+    rc = orc().orc_consistency(_ptr(D, _f32p), ctypes.byref(cr), ctypes.c_int32(len(nd)), ptrs, cams,
+                               _ptr(tol, _f64p), ctypes.c_double(px_tol), cnt.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise RuntimeError(f"orc_consistency rc={rc}")
+    return cnt
+
+
+def fuse(order, depths, cams, rgbs, nbrs, rel_tol, px_tol, min_views, stride, used=None, cap=None):
+    """Restatement of densify._fuse_loop (densify.py:688-762).  depths / cams / rgbs: per view
+    (None where a view has no depth map); nbrs (n_views, max_nb) int32 with -1 for none (and for
+    neighbours without a depth map); rel_tol (n_views, max_nb).  Returns (pts, cols, nv, owner,
+    used maps)."""
+    n = len(depths)
+    order = np.ascontiguousarray(order, np.int32)
+    dd = [None if d is None else _c32(d) for d in depths]
+    rg = [None if r is None else np.ascontiguousarray(r, np.uint8) for r in rgbs]
+    if used is None:
+        used = [None if d is None else np.zeros(d.shape, np.uint8) for d in dd]
+    camv = (OrcCam * n)(*[orc_cam(c) if c is not None else OrcCam() for c in cams])
+    vp = ctypes.c_void_p * n
+    dp = vp(*[0 if d is None else d.ctypes.data for d in dd])
+    rp = vp(*[0 if r is None else r.ctypes.data for r in rg])
+    up = vp(*[0 if u is None else u.ctypes.data for u in used])
+    nbrs = np.ascontiguousarray(nbrs, np.int32)
+    tol = np.ascontiguousarray(rel_tol, np.float64)
+    if cap is None:
+        cap = sum(int(np.ceil(d.shape[0] / stride) * np.ceil(d.shape[1] / stride)) for d in dd if d is not None)
+    pts = np.zeros((cap, 3), np.float64)
+    cols = np.zeros((cap, 3), np.uint8)
+    nv = np.zeros(cap, np.uint8)
+    owner = np.zeros(cap, np.int32)
+    L = orc()
+    L.orc_fuse.restype = ctypes.c_int64
+    tot = L.orc_fuse(ctypes.c_int32(len(order)), _ptr(order, _i32p), ctypes.c_int32(n), dp, camv, rp, up,
+                     _ptr(nbrs, _i32p), ctypes.c_int32(nbrs.shape[1]), _ptr(tol, _f64p), ctypes.c_double(px_tol),
+                     ctypes.c_int32(min_views), ctypes.c_int32(stride), _ptr(pts, _f64p), _ptr(cols, _u8p),
+                     _ptr(nv, _u8p), _ptr(owner, _i32p), ctypes.c_int64(cap))
+    if tot < 0:
+        raise RuntimeError(f"orc_fuse rc={tot}")
+    m = min(tot, cap)
+    return pts[:m], cols[:m], nv[:m], owner[:m], used

@@ -53,6 +53,15 @@ class OmPmView(ctypes.Structure):
     ]
 
 
+class OmCam(ctypes.Structure):
+    _fields_ = [("R", c_double * 9), ("C", c_double * 3), ("f", c_double), ("k1", c_double), ("k2", c_double),
+                ("k3", c_double), ("cx", c_double), ("cy", c_double), ("w", c_int32), ("h", c_int32)]
+
+
+class OmFuseView(ctypes.Structure):
+    _fields_ = [("depth", c_void_p), ("rgb", c_void_p), ("used", c_void_p), ("cam", OmCam)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/om355.h
 SIGNATURES = {
     "om_version": (c_int32, []),
@@ -115,6 +124,12 @@ SIGNATURES = {
     "om_sweep_multi_bytes": (c_size_t, [c_int32]),
     "om_sweep_multi": (c_int32, [ctypes.POINTER(OmSweepDesc), c_int32, ctypes.POINTER(c_void_p), c_int32, c_void_p,
                                  c_size_t, c_void_p, c_void_p]),
+    "om_consistency": (c_int32, [c_void_p, ctypes.POINTER(OmCam), c_int32, ctypes.POINTER(c_void_p),
+                                 ctypes.POINTER(OmCam), ctypes.POINTER(c_double), c_double, c_void_p, c_void_p]),
+    "om_fuse_workspace_bytes": (c_size_t, [ctypes.POINTER(OmFuseView), c_int32, c_int32]),
+    "om_fuse": (c_int32, [c_int32, ctypes.POINTER(c_int32), c_int32, ctypes.POINTER(OmFuseView),
+                          ctypes.POINTER(c_int32), c_int32, ctypes.POINTER(c_double), c_double, c_int32, c_int32,
+                          c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_size_t, c_void_p]),
 }
 
 _lib = None

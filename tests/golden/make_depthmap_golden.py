@@ -48,9 +48,19 @@ def build_reference_package(ref_root: str, out_dir: str) -> str:
     return out_dir
 
 
+def _rotation(rng, deg: float) -> np.ndarray:
+    """Rz(c) Ry(b) Rx(a) with a, b, c uniform in [-deg, deg] degrees."""
+    a, b, c = np.radians(rng.uniform(-deg, deg, 3))
+    Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    Ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    Rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
 def synthetic_survey(seed: int, W: int, H: int, f: float, alt: float, n_cam_x: int, n_cam_y: int, spacing: float,
-                     n_pts: int = 4000):
-    """Nadir grid of cameras over the seeded terrain; tie points = terrain points seen by >= 2 views."""
+                     n_pts: int = 4000, rot_deg: float = 0.0):
+    """Grid of cameras over the seeded terrain, nadir or (rot_deg > 0) with each camera turned by a
+    seeded small rotation; tie points = terrain points seen by >= 2 views."""
     sys.path.insert(0, ROOT)
     import torch
     from py_ortho_amd import scene
@@ -61,6 +71,10 @@ def synthetic_survey(seed: int, W: int, H: int, f: float, alt: float, n_cam_x: i
     ext = (min(xs) - half_w, max(xs) + half_w, min(ys) - half_h, max(ys) + half_h)
     terr = scene.make_terrain(seed, ext)
     cams = [scene.pinhole((x, y, alt), f, W, H) for x, y in centres]
+    if rot_deg > 0:
+        rrng = np.random.default_rng(seed + 300)
+        for cam in cams:
+            cam.R = _rotation(rrng, rot_deg) @ scene.NADIR
     imgs, deps = [], []
     for k, cam in enumerate(cams):
         img, dep = scene.render(terr, cam, noise_seed=seed * 1000 + k, device="cpu")
