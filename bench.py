@@ -860,12 +860,135 @@ def cpu_baseline_view(ref, mu, sd, ray, srcs, geo, par, Cr, deps, depth0, normal
                       f"{ws:.1f} s + {wp:.1f} s wall"}
 
 
+# ------------------------------------------------------------------------------- fusion
+def fuse_flops_per_candidate(S: int) -> int:
+    """Algorithmic FLOPs of one fusion candidate (each IEEE op 1, FMA 2, divide 1): reference point
+    (undistort 7, scale 2, rotate + centre 18) + final average 3, and per neighbour 109: rotate into
+    j 18, project 16, round 2, undistort 7, back-project 20, rotate into the reference 18, project
+    16, agreement 9, accumulate 3."""
+    return 30 + 109 * S
+
+
+def _survey_fuse(torch, cache, scale):
+    """The fusion block and its depth maps (rendered on the device or loaded from `cache`)."""
+    from py_ortho_amd import scene
+    if cache and os.path.exists(cache):
+        z = np.load(cache)
+        cams, counts, ext = scene.survey_cameras(11, 8, 6, 90.0, [(int(round(1600 * scale)), int(round(1200 * scale)))],
+                                                 0.9375, 0.7, 0.6, (4, 4))
+        nbrs = scene.neighbours_by_distance(np.stack([c.C for c in cams]), 90.0, counts)
+        return cams, nbrs, [z[f"d{k}"] for k in range(len(cams))], [z[f"m{k}"] for k in range(len(cams))]
+    sv = scene.config_fuse(device="cuda", scale=scale)
+    cams = [v.cam for v in sv.views]
+    depths = scene.noisy_depths(sv, 12)
+    med = [float(np.median(v.depth.cpu().numpy())) for v in sv.views]
+    if cache:
+        np.savez(cache, **{f"d{k}": d for k, d in enumerate(depths)}, **{f"m{k}": np.float64(m) for k, m in enumerate(med)})
+    return cams, sv.nbrs, depths, med
+
+
+def run_fuse(args, torch):
+    from py_ortho_amd import _lib, fusion
+    lib = _lib.lib()
+    cams, nbrs, depths, med = _survey_fuse(torch, args.scene_cache, args.scale)
+    n = len(cams)
+    px_tol, min_views, stride = 1.5, 3, 2
+    tol = {k: {j: float(np.clip(px_tol * float(med[k]) / max(cams[k].f * float(np.linalg.norm(cams[k].C - cams[j].C)),
+                                                             1e-9), 0.01, 0.05)) for j in nbrs[k]} for k in range(n)}
+    rng = np.random.default_rng(13)
+    rgb_h = [rng.integers(0, 256, (c.h, c.w, 3), dtype=np.uint8) for c in cams]
+    dm = {k: torch.from_numpy(depths[k]).cuda() for k in range(n)}
+    rgb = {k: torch.from_numpy(rgb_h[k]).cuda() for k in range(n)}
+    order = list(range(n))
+    fz = fusion.Fuser(order, dm, cams, nbrs, set(order), lambda k: tol[k], px_tol, min_views, stride, rgb)
+
+    def step():
+        fz.run(reset_used=True)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    prof_read(lib, "fuse_eval")
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    lib.om_profile_enable(0)
+    ev_ms, ev_n = prof_read(lib, "fuse_eval")
+    lib.om_profile_read(None, None, None)
+    P, C, N, O = fz.result()
+    npts, ncand = len(P), fz.evaluated()
+    ms = wall / args.steps * 1e3
+    S = int(np.median([len(x) for x in nbrs]))
+    ev_s = ev_ms / 1e3 / max(args.steps, 1)          # eval time of one step (all views)
+    flops = ncand * fuse_flops_per_candidate(S)
+    # consistency of every view (the check of the repair pass, densify.py:617)
+    cons = []
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    for k in order:
+        cons.append(fusion.consistency(k, dm[k], nbrs[k], dm, cams, px_tol, tol[k]))
+    torch.cuda.synchronize()
+    cons_s = time.perf_counter() - tc
+    pix = fz.pixels
+    res = {
+        "metric": "depth-map Mpixel/s fused on 1 GPU (sequential geometric-consistency fusion)",
+        "value": pix / (ms / 1e3) / 1e6, "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"fusion of a {n}-view block, {cams[0].w}x{cams[0].h} depth maps (rendered depth "
+                               "x (1 + N(0, 2e-3)), 3% + patch holes), 4 neighbours, px_tol 1.5, min_views 3, "
+                               "point_stride 2", "views": n, "W": cams[0].w, "H": cams[0].h, "neighbours": S},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "points": npts, "candidates": ncand, "points_per_s": npts / (ms / 1e3),
+        "consistency_ms_all_views": cons_s * 1e3,
+        "roofline": {"bound": "mfma", "engine": "VALU FP64", "achieved": flops / ev_s / 1e12 if ev_s else 0,
+                     "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": (flops / ev_s / FP64_PEAK) if ev_s else 0,
+                     "traffic": profiled_traffic("fuse_eval_block.md"), "kernel": "fuse_eval_kernel",
+                     "kernel_avg_ms": ev_ms / max(ev_n, 1), "launches_per_step": ev_n / max(args.steps, 1)},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_fuse(order, depths, cams, rgb_h, nbrs, tol, px_tol, min_views, stride,
+                                                (P, C, N, O), args)
+    return res
+
+
+def cpu_baseline_fuse(order, depths, cams, rgb, nbrs, tol, px_tol, min_views, stride, gpu_out, args):
+    """The oracle (C restatement of densify._fuse_loop, one core) over the first views of the
+    order, about 10 s of work; its points must equal the GPU's first points bit for bit."""
+    import oracle
+    n = len(cams)
+    nb = np.full((n, max(len(x) for x in nbrs)), -1, np.int32)
+    tl = np.zeros(nb.shape)
+    for k in range(n):
+        for s_, j in enumerate(nbrs[k]):
+            nb[k, s_], tl[k, s_] = j, tol[k][j]
+    t0 = time.perf_counter()                 # one view, to size a prefix of about 10 s
+    oracle.fuse(order[:1], depths, cams, rgb, nb, tl, px_tol, min_views, stride)
+    m = max(1, min(n, int(10.0 / max(time.perf_counter() - t0, 1e-3))))
+    t0 = time.perf_counter()
+    pts, cols, nv, owner, _ = oracle.fuse(order[:m], depths, cams, rgb, nb, tl, px_tol, min_views, stride)
+    dt = time.perf_counter() - t0
+    P, C, N, O = (x.cpu().numpy() for x in gpu_out)
+    same = (len(pts) <= len(P) and np.array_equal(pts.view(np.uint64), P[:len(pts)].view(np.uint64))
+            and np.array_equal(nv, N[:len(pts)]) and np.array_equal(owner, O[:len(pts)]))
+    pix = sum(int(cams[k].h) * int(cams[k].w) for k in order[:m])
+    return {"value": pix / dt / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
+            "sample": f"the first {m} of {n} views in fusion order ({dt:.1f} s)",
+            "gpu_points_identical": bool(same)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "fuse"], default="c2")
     ap.add_argument("--fmt", choices=["f16", "bf16"], default="f16", help="C4: image storage format")
     ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
     ap.add_argument("--duration", type=float, default=3.0, help="C5: seconds of arrivals")
@@ -874,7 +997,7 @@ def main(argv=None):
     ap.add_argument("--plain-requests", type=int, default=24, help="C5: requests timed one at a time")
     ap.add_argument("--active-mpix", type=float, default=48.0,
                     help="C5: megapixels of admitted requests in flight (latency under overload)")
-    ap.add_argument("--scale", type=float, default=1.0, help="C3: image size scale (1 = the full block)")
+    ap.add_argument("--scale", type=float, default=1.0, help="C3, fuse: image size scale (1 = the full block)")
     ap.add_argument("--plain-per-hyp", action="store_true",
                     help="C3: also time the per-view form with one sweep launch per plane")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -899,7 +1022,8 @@ def main(argv=None):
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, torch)
+    res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
+           "fuse": run_fuse}[args.workload](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -261,10 +261,12 @@ size_t om_fuse_workspace_bytes(const om_fuse_view *views, int32_t n_views, int32
  * array).  nbrs (n_views, max_nb) host int32 neighbour lists, -1 for none (list a view only if it has
  * a depth map, as `j in have` does); rel_tol (n_views, max_nb) host float64.  Appends the fused
  * points in the reference's order: pts (cap, 3) float64, cols (cap, 3) uint8 (NULL to skip),
- * nv (cap,) uint8 = min(views, 255), owner (cap,) int32 = the reference view.  *total (device
- * int64) = the number of points when the stream reaches the end of the call; only the first cap
- * are written, so total > cap means the output was too small.  Updates the views' used maps.
- * Two launches per view on `stream`; the host arrays are read before the call returns. */
+ * nv (cap,) uint8 = min(views, 255), owner (cap,) int32 = the reference view.  total (device,
+ * 2 int64): total[0] = the number of points when the stream reaches the end of the call (only the
+ * first cap are written, so total[0] > cap means the output was too small), total[1] = the grid
+ * pixels evaluated (finite and not used).  Updates the views' used maps.
+ * Two launches per view on `stream` (three above 2048 x 256 grid pixels); the host arrays are read
+ * before the call returns. */
 int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, const om_fuse_view *views, const int32_t *nbrs,
             int32_t max_nb, const double *rel_tol, double px_tol, int32_t min_views, int32_t stride, double *pts,
             uint8_t *cols, uint8_t *nv, int32_t *owner, int64_t cap, int64_t *total, void *workspace,

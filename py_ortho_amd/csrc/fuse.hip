@@ -14,8 +14,9 @@
 // Arithmetic: FP64 in the reference's order (undistort, project, hypot, compare), 3x3 products as
 // NumPy's OpenBLAS dgemm forms them (s = a0*b0; s = fma(a1, b1, s); s = fma(a2, b2, s)).  Points are
 // emitted in the reference's order (views in the given order, grid pixels row-major) through a
-// per-view scan of block counts: the eval kernel's last block turns the counts into offsets, the
-// emit kernel writes each kept point at its offset.
+// per-view scan of block counts: eval counts the kept points of each block; emit adds up the counts
+// of the blocks before its own (or, for views of more than kInlinePrefix blocks, a one-workgroup
+// scan kernel does) and writes each kept point at its offset.
 //
 // Layout: depth (h, w) float32, rgb (h, w, 3) uint8, used flags on the stride grid
 // (ceil(h / stride), ceil(w / stride)) uint8 -- the reference's used map is only ever set and read
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void consistency_kernel(ConsK p)
 }
 
 // ------------------------------------------------------------------------------------------
-// fusion: eval (points, marks, per-block counts, offsets) + emit (ordered compaction)
+// fusion: eval (points, marks, per-block counts) + emit (offsets, ordered compaction)
 // ------------------------------------------------------------------------------------------
 struct FuseNb {
     const float *depth;
@@ -190,8 +191,9 @@ struct FuseWs {
     double *px, *py, *pz;   // per grid pixel: the averaged point when kept
     int32_t *blkcnt;        // kept points per block
     int64_t *blkoff;        // output offset of each block
-    int32_t *ticket;        // blocks done (last block scans; reset to 0 by it)
-    int64_t *total;         // points so far (the running output offset)
+    int64_t *bases;         // output offset of the current view's first point (2, alternating)
+    int64_t *total;         // total[0]: points so far; total[1]: candidates evaluated (finite,
+                            // unused grid pixels)
 };
 
 struct FuseK {
@@ -204,30 +206,33 @@ struct FuseK {
     FuseNb nb[kMaxNb];
 };
 
-__global__ void fuse_init_kernel(int32_t *ticket, int64_t *total)
+__global__ void fuse_init_kernel(int64_t *bases, int64_t *total)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *ticket = 0;
-        *total = 0;
+        bases[0] = 0;
+        total[0] = 0;
+        total[1] = 0;
     }
 }
 
+// One thread per grid pixel.  (A view of 1600 x 1200 at stride 2 is 1875 workgroups, about seven
+// waves per SIMD; splitting a pixel's neighbours over the waves of a workgroup, with the ordered
+// sum through LDS, measured slower: 49 against 33 microseconds per view.)
 __global__ __launch_bounds__(kBlock) void fuse_eval_kernel(FuseK p)
 {
     __shared__ int32_t smark[kMaxNb][kBlock];
-    __shared__ int64_t sscan[kBlock];
-    __shared__ int s_last;
     const int tid = threadIdx.x;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int64_t G = (int64_t)p.gh * p.gw;
     const int64_t g = (int64_t)lb * kBlock + tid;
-    int kept = 0;
+    int kept = 0, cand = 0;
     if (g < G) {
         const int gv = (int)(g / p.gw), gu = (int)(g - (int64_t)gv * p.gw);
         const int v = gv * p.st, u = gu * p.st;
         const float d = p.depth[(int64_t)v * p.cam.w + u];
         uint8_t kc = 0;
         if (isfinite(d) && !p.used[g]) {
+            cand = 1;
             double X[3];
             ref_point((double)d, (double)u, (double)v, p.cam, X);
             double a0 = X[0], a1 = X[1], a2 = X[2];
@@ -261,45 +266,73 @@ __global__ __launch_bounds__(kBlock) void fuse_eval_kernel(FuseK p)
         }
         p.ws.keep[g] = kc;
     }
+    const int ncand = __syncthreads_count(cand);
     const int count = __syncthreads_count(kept);
     if (tid == 0) {
-        __hip_atomic_store(&p.ws.blkcnt[lb], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        s_last = atomicAdd(p.ws.ticket, 1) == p.nblk - 1;
+        p.ws.blkcnt[lb] = count;
+        if (ncand)
+            atomicAdd((unsigned long long *)&p.ws.total[1], (unsigned long long)ncand);
     }
-    __syncthreads();
-    if (!s_last)
-        return;
-    // last block: exclusive scan of the block counts, offset by the points of earlier views
-    __threadfence();
-    int64_t carry = __hip_atomic_load(p.ws.total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int base = 0; base < p.nblk; base += kBlock) {
+}
+
+// exclusive scan of one view's block counts into output offsets after the points of earlier views
+// (bases[par]); writes the next view's base.  One workgroup, for views with many blocks; smaller
+// views take their block offsets inside the emit kernel.
+constexpr int kScan = 1024;
+constexpr int kInlinePrefix = 2048;   // up to this many blocks, emit sums the counts before it
+
+__global__ __launch_bounds__(kScan) void fuse_scan_kernel(FuseWs ws, int32_t nblk, int32_t par)
+{
+    __shared__ int64_t wtot[kScan / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int64_t carry = ws.bases[par];
+    for (int base = 0; base < nblk; base += kScan) {
         const int i = base + tid;
-        const int64_t x =
-            i < p.nblk ? (int64_t)__hip_atomic_load(&p.ws.blkcnt[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-        sscan[tid] = x;
-        __syncthreads();
-        for (int o = 1; o < kBlock; o <<= 1) {   // Hillis-Steele inclusive scan
-            const int64_t y = tid >= o ? sscan[tid - o] : 0;
-            __syncthreads();
-            sscan[tid] += y;
-            __syncthreads();
+        const int64_t x = i < nblk ? ws.blkcnt[i] : 0;
+        int64_t y = x;                               // inclusive scan inside the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t z = __shfl_up(y, o, 64);
+            if (lane >= o)
+                y += z;
         }
-        if (i < p.nblk)
-            p.ws.blkoff[i] = carry + sscan[tid] - x;
-        carry += sscan[kBlock - 1];
+        if (lane == 63)
+            wtot[wave] = y;
+        __syncthreads();
+        int64_t before = 0, all = 0;
+        for (int w = 0; w < kScan / 64; ++w) {
+            before += w < wave ? wtot[w] : 0;
+            all += wtot[w];
+        }
+        if (i < nblk)
+            ws.blkoff[i] = carry + before + y - x;
+        carry += all;
         __syncthreads();
     }
     if (tid == 0) {
-        __hip_atomic_store(p.ws.total, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(p.ws.ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ws.bases[par ^ 1] = carry;
+        ws.total[0] = carry;
     }
+}
+
+// sum over the kBlock threads of a workgroup (every thread gets it)
+__device__ __forceinline__ int64_t block_sum(int64_t x, int64_t *sh)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        x += __shfl_down(x, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        sh[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int64_t s = 0;
+    for (int w = 0; w < kBlock / 64; ++w)
+        s += sh[w];
+    return s;
 }
 
 struct EmitK {
     FuseWs ws;
     const uint8_t *rgb;
-    int32_t W, gh, gw, st, owner;
+    int32_t W, gh, gw, st, owner, nblk, par, inline_prefix;
     int64_t cap;
     double *pts;
     uint8_t *cols, *nv;
@@ -309,6 +342,7 @@ struct EmitK {
 __global__ __launch_bounds__(kBlock) void fuse_emit_kernel(EmitK e)
 {
     __shared__ int wsum[kBlock / 64];
+    __shared__ int64_t red[kBlock / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int64_t G = (int64_t)e.gh * e.gw;
@@ -317,10 +351,30 @@ __global__ __launch_bounds__(kBlock) void fuse_emit_kernel(EmitK e)
     const uint64_t bal = __ballot(kc != 0);
     if (lane == 0)
         wsum[wave] = __popcll(bal);
+    int64_t boff;
+    if (e.inline_prefix) {
+        // this block's offset: the view's base plus the kept points of the blocks before it
+        int64_t x = 0;
+        for (int i = tid; i < lb; i += kBlock)
+            x += e.ws.blkcnt[i];
+        boff = e.ws.bases[e.par] + block_sum(x, red);
+        if (lb == 0) {                               // the next view's base
+            int64_t y = 0;
+            for (int i = tid; i < e.nblk; i += kBlock)
+                y += e.ws.blkcnt[i];
+            const int64_t next = e.ws.bases[e.par] + block_sum(y, red);
+            if (tid == 0) {
+                e.ws.bases[e.par ^ 1] = next;
+                e.ws.total[0] = next;
+            }
+        }
+    } else {
+        boff = e.ws.blkoff[lb];
+    }
     __syncthreads();
     if (kc == 0)
         return;
-    int64_t off = e.ws.blkoff[lb] + __popcll(bal & ((1ull << lane) - 1));
+    int64_t off = boff + __popcll(bal & ((1ull << lane) - 1));
     for (int w = 0; w < wave; ++w)
         off += wsum[w];
     if (off >= e.cap)
@@ -342,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void fuse_emit_kernel(EmitK e)
 inline int64_t grid_n(int32_t n, int32_t st) { return cdiv(n, st); }
 
 struct WsLayout {
-    size_t keep, px, py, pz, blkcnt, blkoff, ticket, bytes;
+    size_t keep, px, py, pz, blkcnt, blkoff, bases, bytes;
 };
 
 inline size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -364,8 +418,8 @@ inline WsLayout ws_layout(int64_t maxG)
     o = up256(o + 4 * (size_t)nblk);
     L.blkoff = o;
     o = up256(o + 8 * (size_t)nblk);
-    L.ticket = o;
-    o = up256(o + 4);
+    L.bases = o;
+    o = up256(o + 16);
     L.bytes = o;
     return L;
 }
@@ -475,11 +529,11 @@ extern "C" int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, c
     ws.pz = reinterpret_cast<double *>(w + L.pz);
     ws.blkcnt = reinterpret_cast<int32_t *>(w + L.blkcnt);
     ws.blkoff = reinterpret_cast<int64_t *>(w + L.blkoff);
-    ws.ticket = reinterpret_cast<int32_t *>(w + L.ticket);
+    ws.bases = reinterpret_cast<int64_t *>(w + L.bases);
     ws.total = total;
     hipStream_t s = as_stream(stream);
     ProfScope ps("fuse", s);
-    hipLaunchKernelGGL(fuse_init_kernel, dim3(1), dim3(64), 0, s, ws.ticket, ws.total);
+    hipLaunchKernelGGL(fuse_init_kernel, dim3(1), dim3(64), 0, s, ws.bases, ws.total);
     if (int rc = launch_check("om_fuse init"))
         return rc;
     for (int32_t o = 0; o < n_order; ++o) {
@@ -508,7 +562,13 @@ extern "C" int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, c
         }
         const int64_t G = (int64_t)p.gh * p.gw;
         p.nblk = (int32_t)cdiv(G, kBlock);
-        hipLaunchKernelGGL(fuse_eval_kernel, dim3((unsigned)p.nblk), dim3(kBlock), 0, s, p);
+        {
+            ProfScope pe("fuse_eval", s);
+            hipLaunchKernelGGL(fuse_eval_kernel, dim3((unsigned)p.nblk), dim3(kBlock), 0, s, p);
+        }
+        const int par = o & 1, inl = p.nblk <= kInlinePrefix;
+        if (!inl)
+            hipLaunchKernelGGL(fuse_scan_kernel, dim3(1), dim3(kScan), 0, s, ws, p.nblk, par);
         EmitK e{};
         e.ws = ws;
         e.rgb = V.rgb;
@@ -522,6 +582,9 @@ extern "C" int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, c
         e.cols = cols;
         e.nv = nv;
         e.own = owner;
+        e.nblk = p.nblk;
+        e.par = par;
+        e.inline_prefix = inl;
         hipLaunchKernelGGL(fuse_emit_kernel, dim3((unsigned)p.nblk), dim3(kBlock), 0, s, e);
         if (int rc = launch_check("om_fuse"))
             return rc;

@@ -86,6 +86,108 @@ def used_full(used_grid, h: int, w: int, stride: int) -> np.ndarray:
     return out
 
 
+class Fuser:
+    """One prepared om_fuse call over fixed views: the view table, neighbour table, used maps,
+    output buffers and workspace are built once; run() enqueues the whole sequential fusion.
+
+    Arguments as fuse(); used: k -> grid used map, or None for fresh zeroed maps (run() with
+    reset_used=True zeroes them again before fusing, as a new densify run starts from zero)."""
+
+    def __init__(self, order, dm: Mapping, cams, nbrs, have, rel_tols: Callable, px_tol, min_views, stride=1,
+                 rgb=None, used=None, cap: int | None = None):
+        stride = max(1, int(stride or 1))
+        order = [int(k) for k in order]
+        if not order:
+            raise ValueError("Fuser: empty view order")
+        views = sorted(set(order) | {j for k in order for j in nbrs[k] if j in have})
+        dev = dm[order[0]].device
+        self._used_buf = None
+        if used is None:                 # all grid maps in one allocation: one fill clears them
+            shapes = {k: grid_shape(cams[k].h, cams[k].w, stride) for k in views}
+            self._used_buf = torch.zeros(sum(a * b for a, b in shapes.values()), dtype=torch.uint8, device=dev)
+            used, o = {}, 0
+            for k in views:
+                a, b = shapes[k]
+                used[k] = self._used_buf[o:o + a * b].view(a, b)
+                o += a * b
+        n_views = max(views) + 1
+        V = (_lib.OmFuseView * n_views)()
+        self._keep = []
+        for k in views:
+            cam = cams[k]
+            d = _dev_f32(dm[k], int(cam.h), int(cam.w), f"fuse: dm[{k}]")
+            u = used[k]
+            if tuple(u.shape) != grid_shape(cam.h, cam.w, stride) or u.dtype != torch.uint8 or not u.is_contiguous():
+                raise ValueError(f"fuse: used[{k}] must be a contiguous uint8 grid map of shape "
+                                 f"{grid_shape(cam.h, cam.w, stride)}")
+            V[k].depth, V[k].used, V[k].cam = d.data_ptr(), u.data_ptr(), om_cam(cam)
+            self._keep.append(d)
+            if rgb is not None and k in order:
+                c = rgb[k]
+                if not (isinstance(c, torch.Tensor) and c.is_cuda and c.dtype == torch.uint8 and c.is_contiguous()
+                        and tuple(c.shape) == (int(cam.h), int(cam.w), 3)):
+                    raise ValueError(f"fuse: rgb[{k}] must be a contiguous (h, w, 3) uint8 device tensor")
+                V[k].rgb = c.data_ptr()
+                self._keep.append(c)
+        max_nb = max(1, max(len(nbrs[k]) for k in order))
+        nb = np.full((n_views, max_nb), -1, np.int32)
+        tol = np.zeros((n_views, max_nb), np.float64)
+        for k in order:
+            t = rel_tols(k)
+            for s, j in enumerate(nbrs[k]):
+                if j in have:
+                    nb[k, s] = j
+                    tol[k, s] = float(t[j])
+        self.candidates = sum(grid_shape(cams[k].h, cams[k].w, stride)[0] * grid_shape(cams[k].h, cams[k].w, stride)[1]
+                              for k in order)
+        self.pixels = sum(int(cams[k].h) * int(cams[k].w) for k in order)
+        if cap is None:
+            cap = self.candidates
+        L = _lib.lib()
+        self.ws_bytes = L.om_fuse_workspace_bytes(V, n_views, stride)
+        self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
+        self.P = torch.empty((cap, 3), dtype=torch.float64, device=dev)
+        self.Cl = torch.empty((cap, 3), dtype=torch.uint8, device=dev) if rgb is not None else None
+        self.N = torch.empty(cap, dtype=torch.uint8, device=dev)
+        self.O = torch.empty(cap, dtype=torch.int32, device=dev)
+        self.tot = torch.zeros(2, dtype=torch.int64, device=dev)
+        self.used, self.views, self.cap, self.device = used, views, int(cap), dev
+        self._V, self._nb, self._tol = V, nb, tol
+        self._order = np.ascontiguousarray(order, np.int32)
+        self._args = (float(px_tol), int(min_views), stride, max_nb, n_views)
+
+    def run(self, stream=None, reset_used: bool = False) -> None:
+        """Enqueue the fusion on `stream` (torch's current stream by default)."""
+        px_tol, min_views, stride, max_nb, n_views = self._args
+        if reset_used:
+            if self._used_buf is not None:
+                self._used_buf.zero_()
+            else:
+                for k in self.views:
+                    self.used[k].zero_()
+        st = stream if stream is not None else torch.cuda.current_stream(self.device).cuda_stream
+        rc = _lib.lib().om_fuse(len(self._order), self._order.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_views,
+                                self._V, self._nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_nb,
+                                self._tol.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), px_tol, min_views, stride,
+                                self.P.data_ptr(), self.Cl.data_ptr() if self.Cl is not None else None,
+                                self.N.data_ptr(), self.O.data_ptr(), self.cap, self.tot.data_ptr(),
+                                self.ws.data_ptr(), self.ws_bytes, st)
+        _lib.check(rc, "om_fuse")
+
+    def evaluated(self) -> int:
+        """Grid pixels the last run evaluated (finite and not used when their view came up)."""
+        torch.cuda.synchronize(self.device)
+        return int(self.tot[1].item())
+
+    def result(self):
+        """(pts, cols or None, nv, owner) device tensors of the last run (synchronises)."""
+        torch.cuda.synchronize(self.device)
+        n = int(self.tot[0].item())
+        if n > self.cap:
+            raise RuntimeError(f"fuse: {n} points exceed the output capacity {self.cap}")
+        return self.P[:n], (self.Cl[:n] if self.Cl is not None else None), self.N[:n], self.O[:n]
+
+
 def fuse(order: Sequence[int], dm: Mapping, used, cams, nbrs, have, rel_tols: Callable, opt, px_tol, min_views,
          rgb, pts: list, cols: list, nv: list, owner: list, cap: int | None = None, stream=None):
     """densify._fuse_loop (densify.py:688-762) on the device.
@@ -96,67 +198,15 @@ def fuse(order: Sequence[int], dm: Mapping, used, cams, nbrs, have, rel_tols: Ca
     colours.  Appends one tensor each to pts (n, 3) float64, cols (n, 3) uint8, nv (n,) uint8 and
     owner (n,) int32 (the reference appends one per view; the concatenation is the same) and
     returns the used maps."""
-    stride = max(1, int(getattr(opt, "point_stride", 1) or 1))
-    order = [int(k) for k in order]
-    views = sorted(set(order) | {j for k in order for j in nbrs[k] if j in have})
-    if not order:
+    if not list(order):
         return used
-    dev = dm[order[0]].device
-    if used is None:
-        used = new_used({k: dm[k] for k in views}, cams, stride, dev)
-    n_views = max(views) + 1
-    V = (_lib.OmFuseView * n_views)()
-    keep = []
-    for k in views:
-        cam = cams[k]
-        d = _dev_f32(dm[k], int(cam.h), int(cam.w), f"fuse: dm[{k}]")
-        u = used[k]
-        if tuple(u.shape) != grid_shape(cam.h, cam.w, stride) or u.dtype != torch.uint8 or not u.is_contiguous():
-            raise ValueError(f"fuse: used[{k}] must be a contiguous uint8 grid map of shape "
-                             f"{grid_shape(cam.h, cam.w, stride)}")
-        V[k].depth, V[k].used, V[k].cam = d.data_ptr(), u.data_ptr(), om_cam(cam)
-        if rgb is not None and k in order:
-            c = rgb[k]
-            if not (isinstance(c, torch.Tensor) and c.is_cuda and c.dtype == torch.uint8 and c.is_contiguous()
-                    and tuple(c.shape) == (int(cam.h), int(cam.w), 3)):
-                raise ValueError(f"fuse: rgb[{k}] must be a contiguous (h, w, 3) uint8 device tensor")
-            V[k].rgb = c.data_ptr()
-            keep.append(c)
-    max_nb = max(1, max(len(nbrs[k]) for k in order))
-    nb = np.full((n_views, max_nb), -1, np.int32)
-    tol = np.zeros((n_views, max_nb), np.float64)
-    for k in order:
-        t = rel_tols(k)
-        for s, j in enumerate(nbrs[k]):
-            if j in have:
-                nb[k, s] = j
-                tol[k, s] = float(t[j])
-    if cap is None:
-        cap = sum(grid_shape(cams[k].h, cams[k].w, stride)[0] * grid_shape(cams[k].h, cams[k].w, stride)[1]
-                  for k in order)
-    L = _lib.lib()
-    wsb = L.om_fuse_workspace_bytes(V, n_views, stride)
-    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
-    P = torch.empty((cap, 3), dtype=torch.float64, device=dev)
-    Cl = torch.empty((cap, 3), dtype=torch.uint8, device=dev) if rgb is not None else None
-    N = torch.empty(cap, dtype=torch.uint8, device=dev)
-    O = torch.empty(cap, dtype=torch.int32, device=dev)
-    tot = torch.zeros(1, dtype=torch.int64, device=dev)
-    ordv = np.ascontiguousarray(order, np.int32)
-    st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
-    rc = L.om_fuse(len(order), ordv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_views, V,
-                   nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_nb,
-                   tol.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(px_tol), int(min_views), stride,
-                   P.data_ptr(), Cl.data_ptr() if Cl is not None else None, N.data_ptr(), O.data_ptr(), int(cap),
-                   tot.data_ptr(), ws.data_ptr(), wsb, st)
-    _lib.check(rc, "om_fuse")
-    n = int(tot.item())
-    if n > cap:
-        raise RuntimeError(f"fuse: {n} points exceed the output capacity {cap}")
-    pts.append(P[:n])
-    if Cl is not None:
-        cols.append(Cl[:n])
-    nv.append(N[:n])
-    owner.append(O[:n])
-    del keep
-    return used
+    f = Fuser(order, dm, cams, nbrs, have, rel_tols, px_tol, min_views, getattr(opt, "point_stride", 1), rgb, used,
+              cap)
+    f.run(stream)
+    P, C, N, O = f.result()
+    pts.append(P)
+    if C is not None:
+        cols.append(C)
+    nv.append(N)
+    owner.append(O)
+    return f.used

@@ -314,6 +314,33 @@ def make_survey(seed: int, cams: Sequence[Camera], counts: Sequence[int], ext, a
     return Survey(terr, views, nbrs, dmin=alt - hmax - 2.0, dmax=alt + 2.0)
 
 
+def noisy_depths(survey: Survey, seed: int, rel_noise: float = 2e-3, hole_frac: float = 0.03, n_patches: int = 4):
+    """Depth maps as a matcher leaves them, for fusion: each view's rendered camera-axis depth
+    x (1 + N(0, rel_noise)), with random single-pixel and rectangular NaN holes (unmatched or masked
+    pixels).  -> list of (h, w) float32 numpy arrays."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in survey.views:
+        d = v.depth.cpu().numpy().astype(np.float64)
+        d = (d * (1.0 + rel_noise * rng.standard_normal(d.shape))).astype(np.float32)
+        d[rng.random(d.shape) < hole_frac] = np.nan
+        h, w = d.shape
+        for _ in range(n_patches):
+            y0, x0 = int(rng.integers(h)), int(rng.integers(w))
+            d[y0:y0 + h // 8, x0:x0 + w // 8] = np.nan
+        out.append(d)
+    return out
+
+
+def config_fuse(device="cpu", scale: float = 1.0) -> Survey:
+    """Fusion block (seed 11): 8 x 6 nadir grid at 90 m with 70 % forward / 60 % side overlap, 48
+    views of 1600x1200 (the reference's default matching size, DepthOptions.max_image_dim), f =
+    0.9375 W, 4 nearest views each (DepthOptions.neighbors)."""
+    sizes = [(int(round(1600 * scale)), int(round(1200 * scale)))]
+    cams, counts, ext = survey_cameras(11, 8, 6, 90.0, sizes, 0.9375, 0.7, 0.6, (4, 4))
+    return make_survey(11, cams, counts, ext, 90.0, device=device, march=48, bisect=30, n_bumps=24)
+
+
 def config_c3(device="cpu", scale: float = 1.0) -> Survey:
     """Batched block: 8 x 6 nadir grid at 90 m with 70 % forward / 60 % side overlap, 48 views whose
     sizes are drawn from 800x600 .. 2400x1800 (f = 0.9375 W), S = U{4..12} nearest views (seed 4)."""

@@ -41,20 +41,8 @@ def rel_tols_by_baseline(cams, nbrs, px_tol, depth_med):
 
 
 def noisy_depths(survey, seed, rel_noise=2e-3, hole_frac=0.03, n_patches=4):
-    """Rendered camera-axis depth of each view x (1 + N(0, rel_noise)), with random single-pixel and
-    rectangular NaN holes (unmatched / masked pixels)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for v in survey.views:
-        d = v.depth.cpu().numpy().astype(np.float64)
-        d = (d * (1.0 + rel_noise * rng.standard_normal(d.shape))).astype(np.float32)
-        d[rng.random(d.shape) < hole_frac] = np.nan
-        h, w = d.shape
-        for _ in range(n_patches):
-            y0, x0 = int(rng.integers(h)), int(rng.integers(w))
-            d[y0:y0 + h // 8, x0:x0 + w // 8] = np.nan
-        out.append(d)
-    return out
+    from py_ortho_amd import scene
+    return scene.noisy_depths(survey, seed, rel_noise, hole_frac, n_patches)
 
 
 def survey_case(seed, nx, ny, sizes, counts=(3, 5), k1=0.0, k2=0.0, alt=60.0, device="cpu"):

@@ -151,9 +151,10 @@ def test_fusion_capacity_and_errors(survey):
 
 
 def test_fusion_invariants_at_full_size():
-    """A 1600x1200 3x2 block: every point is averaged from >= min_views agreeing images, owners follow
-    the fusion order, points are finite and lie on the terrain's height range, and a second run
-    from fresh used maps repeats the first bit for bit."""
+    """A 1600x1200 3x2 block: equal to the oracle bit for bit at stride 2 (block offsets summed in
+    the emit kernel) and stride 1 (7500 blocks a view: the scan kernel); every point is averaged
+    from >= min_views agreeing images, owners follow the fusion order, points lie in the terrain's
+    height range, and a second run from fresh used maps repeats the first bit for bit."""
     from py_ortho_amd import fusion, scene
     print("fusion full size 1600x1200", flush=True)
     cams, counts, ext = scene.survey_cameras(9, 3, 2, 90.0, [(1600, 1200)], 0.9375, 0.7, 0.6, (4, 4))
@@ -161,15 +162,23 @@ def test_fusion_invariants_at_full_size():
     depths = fusion_cases.noisy_depths(sv, 10)
     n = len(cams)
     tol = fusion_cases.rel_tols_by_baseline(cams, sv.nbrs, 1.5, 80.0)
-    dm = {k: torch.from_numpy(depths[k]).cuda() for k in range(n)}
+    have = list(range(n))
+    for stride in (2, 1):
+        print(f"  stride {stride}", flush=True)
+        a, ua = _gpu_fuse(have, depths, cams, None, sv.nbrs, have, tol, 1.5, 3, stride)
+        b, ub = _oracle_fuse(have, depths, cams, [np.zeros((c.h, c.w, 3), np.uint8) for c in cams], sv.nbrs, have,
+                             tol, 1.5, 3, stride)
+        assert _same(a[0], b[0]) and _same(a[2], b[2]) and _same(a[3], b[3])
+        assert all(_same(ua[k], ub[k]) for k in have)
+        P, N, O = a[0], a[2], a[3]
+        assert len(P) > 100000 and np.isfinite(P).all()
+        assert (N >= 3).all() and (np.diff(O) >= 0).all()
+        assert P[:, 2].min() > -3.0 and P[:, 2].max() < 18.0
+    dm = {k: torch.from_numpy(depths[k]).cuda() for k in have}
+    f = fusion.Fuser(have, dm, cams, sv.nbrs, set(have), lambda k: tol[k], 1.5, 3, 2)
     runs = []
     for _ in range(2):
-        pts, cols, nv, owner = [], [], [], []
-        fusion.fuse(list(range(n)), dm, None, cams, sv.nbrs, set(range(n)), lambda k: tol[k],
-                    SimpleNamespace(point_stride=2), 1.5, 3, None, pts, cols, nv, owner)
-        runs.append([x[0].cpu().numpy() for x in (pts, nv, owner)])
-    P, N, O = runs[0]
-    assert len(P) > 100000 and np.isfinite(P).all()
-    assert (N >= 3).all() and (np.diff(O) >= 0).all()
-    assert P[:, 2].min() > -3.0 and P[:, 2].max() < 18.0
+        f.run(reset_used=True)
+        runs.append([x.cpu().numpy().copy() for x in f.result() if x is not None])
     assert all(_same(x, y) for x, y in zip(runs[0], runs[1]))
+    assert f.evaluated() > len(runs[0][0])
