@@ -983,12 +983,105 @@ def cpu_baseline_fuse(order, depths, cams, rgb, nbrs, tol, px_tol, min_views, st
             "gpu_points_identical": bool(same)}
 
 
+# ------------------------------------------------------------------------------- SGM
+def run_sgm(args, torch):
+    """Semi-global matching of the height sweep's coarse level (mvs.py:305-325): B tiles of
+    (D, H, W) = (96, 48, 48) -- a 160-cell tile with its 16-cell pad at level 2 (DenseOptions
+    tile 160, window 3, levels 3) and 96 height planes -- aggregated in one om_sgm call."""
+    from py_ortho_amd import _lib
+    lib = _lib.lib()
+    B, D, H, W = args.sgm_tiles, 96, 48, 48
+    rng = np.random.default_rng(17)
+    # smooth surfaces with cliffs under noise, as 1 - NCC of the sweep gives them
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    cost = np.empty((B, D, H, W), np.float32)
+    for b in range(B):
+        z = 30 + 10 * np.sin(xx / (6 + b % 5)) + 8 * np.cos(yy / (7 + b % 3))
+        z[:, W // 2 + (b % 7):] += 15
+        d = np.arange(D, dtype=np.float32)[:, None, None]
+        cost[b] = np.minimum(np.abs(d - z[None]) * 0.1, 1.0) + rng.uniform(0, 0.4, (D, H, W))
+    c = torch.from_numpy(cost).cuda()
+    agg = torch.empty_like(c)
+    nb = lib.om_sgm_workspace_bytes(B, D, H, W)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    P1, P2 = 0.05, 0.6
+
+    def step():
+        _lib.check(lib.om_sgm(c.data_ptr(), B, D, H, W, P1, P2, agg.data_ptr(), ws.data_ptr(), nb, st), "om_sgm")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    prof_read(lib, "sgm_path")
+    prof_read(lib, "sgm")
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    lib.om_profile_enable(0)
+    path_ms, path_n = prof_read(lib, "sgm_path")
+    all_ms, all_n = prof_read(lib, "sgm")
+    lib.om_profile_read(None, None, None)
+    ms = wall / args.steps * 1e3
+    n = B * D * H * W
+    # one tile per call, as the reference calls sgm once per tile
+    c1 = c[:1].contiguous()
+    a1 = torch.empty_like(c1)
+    nb1 = lib.om_sgm_workspace_bytes(1, D, H, W)
+    ws1 = torch.empty(nb1, dtype=torch.uint8, device="cuda")
+    for _ in range(3):
+        _lib.check(lib.om_sgm(c1.data_ptr(), 1, D, H, W, P1, P2, a1.data_ptr(), ws1.data_ptr(), nb1, st), "om_sgm")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = 50
+    for _ in range(reps):
+        _lib.check(lib.om_sgm(c1.data_ptr(), 1, D, H, W, P1, P2, a1.data_ptr(), ws1.data_ptr(), nb1, st), "om_sgm")
+    torch.cuda.synchronize()
+    single_ms = (time.perf_counter() - t1) / reps * 1e3
+    path_s = path_ms / 1e3 / max(path_n, 1)
+    res = {
+        "metric": "aggregated cost entries/s on 1 GPU (semi-global matching, 8 paths)",
+        "value": n / (ms / 1e3) / 1e6, "unit": "Mentries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"SGM of {B} coarse-level height-sweep tiles, (D, H, W) = ({D}, {H}, {W}), "
+                               "P1 0.05, P2 0.6, one call", "tiles": B, "D": D, "H": H, "W": W},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "one_tile_per_call_ms": single_ms,
+        "roofline": {"bound": "hbm", "achieved": 8.0 * n / (all_ms / 1e3 / max(all_n, 1)) / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": 8.0 * n / (all_ms / 1e3 / max(all_n, 1)) / HBM_PEAK,
+                     "traffic": profiled_traffic("sgm_tiles.md"), "kernel": "om_sgm (transpose, paths, ordered sum)",
+                     "kernel_avg_ms": all_ms / max(all_n, 1), "path_kernel_avg_ms": path_s * 1e3},
+    }
+    if not args.no_cpu:
+        import oracle
+        k = 0
+        t0 = time.perf_counter()
+        while True:
+            ref = oracle.sgm(cost[k % B], P1, P2)
+            k += 1
+            if time.perf_counter() - t0 > 10.0 or k >= B:
+                break
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(ref.view(np.uint32), agg[(k - 1) % B].cpu().numpy().view(np.uint32)))
+        res["cpu_baseline"] = {"value": k * D * H * W / dt / 1e6, "unit": "Mentries/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} tiles one after another ({dt:.1f} s)", "gpu_identical": same}
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "fuse"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "fuse", "sgm"], default="c2")
+    ap.add_argument("--sgm-tiles", type=int, default=256, help="sgm: tiles aggregated in one call")
     ap.add_argument("--fmt", choices=["f16", "bf16"], default="f16", help="C4: image storage format")
     ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
     ap.add_argument("--duration", type=float, default=3.0, help="C5: seconds of arrivals")
@@ -1023,7 +1116,7 @@ def main(argv=None):
         torch.cuda.synchronize()
         del a, b
     res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
-           "fuse": run_fuse}[args.workload](args, torch)
+           "fuse": run_fuse, "sgm": run_sgm}[args.workload](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -19,6 +19,7 @@
  *   om_ref_stats          mu / sd of _sweep                 orthomosaic/densify.py:327-328
  *   om_consistency        densify._consistency              orthomosaic/densify.py:441-478
  *   om_fuse               densify._fuse / _fuse_loop        orthomosaic/densify.py:676-762
+ *   om_sgm                _mvs.sgm                          orthomosaic/_mvs.pyx:98-250
  */
 #ifndef OM355_H
 #define OM355_H
@@ -271,6 +272,14 @@ int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, const om_fus
             int32_t max_nb, const double *rel_tol, double px_tol, int32_t min_views, int32_t stride, double *pts,
             uint8_t *cols, uint8_t *nv, int32_t *owner, int64_t cap, int64_t *total, void *workspace,
             size_t workspace_bytes, void *stream);
+
+/* _mvs.sgm (_mvs.pyx:98-250): semi-global matching of B cost volumes cost (B, D, H, W) float32 ->
+ * agg (B, D, H, W) float32, the sum over the 8 path directions in the reference's order with its
+ * float32 operations (bit-identical).  D <= 512 (OM_EUNSUP above).  workspace: om_sgm_workspace_bytes
+ * (9 volumes: the transposed cost and one path volume per direction). */
+size_t om_sgm_workspace_bytes(int64_t B, int32_t D, int32_t H, int32_t W);
+int om_sgm(const float *cost, int64_t B, int32_t D, int32_t H, int32_t W, float P1, float P2, float *agg,
+           void *workspace, size_t workspace_bytes, void *stream);
 
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);

@@ -399,3 +399,16 @@ def fuse(order, depths, cams, rgbs, nbrs, rel_tol, px_tol, min_views, stride, us
         raise RuntimeError(f"orc_fuse rc={tot}")
     m = min(tot, cap)
     return pts[:m], cols[:m], nv[:m], owner[:m], used
+
+
+# ------------------------------------------------------------------ semi-global matching (om_sgm.c)
+def sgm(cost, P1, P2):
+    """Restatement of _mvs.sgm (_mvs.pyx:98-250): (D, H, W) float32 -> aggregated (D, H, W)."""
+    cost = _c32(cost)
+    D, H, W = cost.shape
+    agg = np.zeros((D, H, W), np.float32)
+    rc = orc().orc_sgm(_ptr(cost, _f32p), ctypes.c_int64(D), ctypes.c_int64(H), ctypes.c_int64(W),
+                       ctypes.c_float(P1), ctypes.c_float(P2), _ptr(agg, _f32p))
+    if rc:
+        raise RuntimeError(f"orc_sgm rc={rc}")
+    return agg

@@ -1,7 +1,8 @@
 """Backend array operations of the ground-grid height sweep on the device (libom355 kernels).
 
 Reference: CPUBackend.sample_view / box / topk_mean / argmin0 / argmax0 (orthomosaic/backend.py:67-99),
-used by mvs.dense_reconstruct's score_at (mvs.py:254-304).
+used by mvs.dense_reconstruct's score_at (mvs.py:254-304), and _mvs.sgm (_mvs.pyx:98-250), its
+cost aggregation on the coarse level (mvs.py:325).
 """
 from __future__ import annotations
 
@@ -72,3 +73,25 @@ def argminmax0(a: torch.Tensor, want_max: bool):
     check(_lib.lib().om_argminmax0(_need(a, "a"), n, M, int(bool(want_max)), idx.data_ptr(), val.data_ptr(),
                                    _stream()), "argmin0/argmax0")
     return idx, val
+
+
+def sgm(cost: torch.Tensor, P1: float, P2: float, workers: int = 1) -> torch.Tensor:
+    """_mvs.sgm (_mvs.pyx:98-250): semi-global aggregation of a (D, H, W) float32 cost volume, or of
+    a batch (B, D, H, W) in one call, along the 8 path directions; bit-identical to the reference.
+    `workers` is accepted for the reference's signature (the device runs every path at once)."""
+    if cost.dim() not in (3, 4):
+        raise ValueError(f"sgm: cost must be (D, H, W) or (B, D, H, W), got {tuple(cost.shape)}")
+    c = cost.contiguous()
+    if c.dtype != torch.float32:
+        raise ValueError("sgm: float32 cost expected")
+    B = int(c.shape[0]) if c.dim() == 4 else 1
+    D, H, W = (int(v) for v in c.shape[-3:])
+    agg = torch.empty_like(c)
+    if c.numel() == 0:
+        return agg.zero_()
+    L = _lib.lib()
+    nb = L.om_sgm_workspace_bytes(B, D, H, W)
+    ws = _workspace(nb, c.device)
+    check(L.om_sgm(_need(c, "cost"), B, D, H, W, float(P1), float(P2), agg.data_ptr(), ws.data_ptr(), nb, _stream()),
+          "sgm")
+    return agg
