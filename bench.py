@@ -1075,12 +1075,88 @@ def run_sgm(args, torch):
     return res
 
 
+# ------------------------------------------------------------------------------- view preparation
+def run_prep(args, torch):
+    """densify's per-view preparation (load_view, densify.py:528-535, and the pyramid of
+    depth_map, :384-386) for 48 decoded 1600x1200 views: depth_view (uint8 RGB x gain + bias ->
+    grey with mask NaN + uint8 colour) and two halvings, inputs resident in HBM; also the rate
+    with each view's upload from pinned host memory."""
+    from py_ortho_amd import _lib, prep
+    lib = _lib.lib()
+    n_views, h, w = 48, 1200, 1600
+    rng = np.random.default_rng(19)
+    rgb_h = [torch.from_numpy(rng.integers(0, 256, (h, w, 3), dtype=np.uint8)).pin_memory() for _ in range(n_views)]
+    mask_h = [torch.from_numpy((rng.random((h, w)) < 0.98).view(np.uint8)).pin_memory() for _ in range(n_views)]
+    rgb_d = [x.cuda() for x in rgb_h]
+    mask_d = [x.cuda() for x in mask_h]
+    gains = [(1.0 + 0.1 * rng.standard_normal(3)).astype(np.float32) for _ in range(n_views)]
+
+    def step(upload: bool):
+        out = []
+        for k in range(n_views):
+            r, m = ((rgb_h[k].cuda(non_blocking=True), mask_h[k].cuda(non_blocking=True)) if upload
+                    else (rgb_d[k], mask_d[k]))
+            out.append(prep.prepare_view(r, gains[k], None, m, levels=3))
+        return out
+
+    for _ in range(args.warmup):
+        step(False)
+        step(True)
+    torch.cuda.synchronize()
+    lib.om_profile_enable(1)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step(False)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    lib.om_profile_enable(0)
+    lib.om_profile_read(None, None, None)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    wall_up = time.perf_counter() - t1
+    ms = wall / args.steps * 1e3
+    pix = n_views * h * w
+    # HBM bytes a pixel: depth_view reads 3 + 1 and writes 4 + 3; the halvings read 4 + 1 and
+    # write 1 + 0.25
+    bpp = 3 + 1 + 4 + 3 + 4 + 1 + 1 + 0.25
+    res = {
+        "metric": "views prepared per second on 1 GPU (matching image + colour + pyramid)",
+        "value": pix / (ms / 1e3) / 1e6, "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n_views} decoded {w}x{h} views: depth_view (gain, mask) + 2 halvings",
+                   "views": n_views, "W": w, "H": h},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+        "with_upload_mpix_s": pix / (wall_up / args.steps) / 1e6,
+        "roofline": {"bound": "hbm", "achieved": bpp * pix / (ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": bpp * pix / (ms / 1e3) / HBM_PEAK, "traffic": None,
+                     "kernel": "depth_view_kernel + half_kernel (step time, launches included)"},
+    }
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 5.0 and k < n_views:
+            g0, _ = oracle.depth_view(rgb_h[k].numpy(), gains[k], None, mask_h[k].numpy().view(bool))
+            oracle.half(oracle.half(g0))
+            k += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": k * h * w / dt / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} views ({dt:.1f} s): NumPy depth_view + C halvings"}
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "fuse", "sgm"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "fuse", "sgm", "prep"], default="c2")
     ap.add_argument("--sgm-tiles", type=int, default=256, help="sgm: tiles aggregated in one call")
     ap.add_argument("--fmt", choices=["f16", "bf16"], default="f16", help="C4: image storage format")
     ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
@@ -1116,7 +1192,7 @@ def main(argv=None):
         torch.cuda.synchronize()
         del a, b
     res = {"c2": run_c2, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
-           "fuse": run_fuse, "sgm": run_sgm}[args.workload](args, torch)
+           "fuse": run_fuse, "sgm": run_sgm, "prep": run_prep}[args.workload](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -20,6 +20,7 @@
  *   om_consistency        densify._consistency              orthomosaic/densify.py:441-478
  *   om_fuse               densify._fuse / _fuse_loop        orthomosaic/densify.py:676-762
  *   om_sgm                _mvs.sgm                          orthomosaic/_mvs.pyx:98-250
+ *   om_depth_view         _fast.depth_view                  orthomosaic/_fast.pyx:577-616
  */
 #ifndef OM355_H
 #define OM355_H
@@ -280,6 +281,13 @@ int om_fuse(int32_t n_order, const int32_t *order, int32_t n_views, const om_fus
 size_t om_sgm_workspace_bytes(int64_t B, int32_t D, int32_t H, int32_t W);
 int om_sgm(const float *cost, int64_t B, int32_t D, int32_t H, int32_t W, float P1, float P2, float *agg,
            void *workspace, size_t workspace_bytes, void *stream);
+
+/* _fast.depth_view (_fast.pyx:577-616), densify's matching image (densify.py:528-535): rgb (h, w, 3)
+ * uint8 device -> grey (h, w) float32 = ((r + g + b) / 3 - 128) / 64 of rgb x gain + bias (NaN where
+ * mask (h, w) uint8 is 0) and rgb8 (h, w, 3) uint8 = clip(v + 0.5, 0, 255).  gain, bias: host arrays
+ * of 3 floats or NULL; mask: device or NULL.  Bit-identical. */
+int om_depth_view(const uint8_t *rgb, int32_t h, int32_t w, const float *gain, const float *bias, const uint8_t *mask,
+                  float *grey, uint8_t *rgb8, void *stream);
 
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);

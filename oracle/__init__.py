@@ -412,3 +412,23 @@ def sgm(cost, P1, P2):
     if rc:
         raise RuntimeError(f"orc_sgm rc={rc}")
     return agg
+
+
+# ------------------------------------------------------------------ matching-image preparation
+def depth_view(rgb, gain=None, bias=None, mask=None):
+    """Restatement of densify.load_view's NumPy path (densify.py:537-547), which the reference's
+    test pairs bit for bit with _fast.depth_view (test_native_parity.py:380-398):
+    -> (grey (h, w) float32, rgb8 (h, w, 3) uint8)."""
+    v = np.asarray(rgb, np.uint8).astype(np.float32)
+    if gain is not None:
+        v *= np.asarray(gain, np.float32)
+    if bias is not None:
+        v += np.asarray(bias, np.float32)
+    s = v[..., 0] + v[..., 1]
+    s = s + v[..., 2]
+    s = (s / np.float32(3.0)).astype(np.float32)
+    grey = np.ascontiguousarray(((s - np.float32(128.0)) / np.float32(64.0)).astype(np.float32))
+    if mask is not None:
+        grey[~np.asarray(mask, bool)] = np.nan
+    rgb8 = np.clip(v + np.float32(0.5), 0, 255).astype(np.uint8)
+    return grey, rgb8
