@@ -21,6 +21,7 @@
  *   om_fuse               densify._fuse / _fuse_loop        orthomosaic/densify.py:676-762
  *   om_sgm                _mvs.sgm                          orthomosaic/_mvs.pyx:98-250
  *   om_depth_view         _fast.depth_view                  orthomosaic/_fast.pyx:577-616
+ *   om_f32_to_f16 / om_f16_to_f32 / om_filter_depth  the depth-map file set  densify.py:590-592, :613-614, :641
  */
 #ifndef OM355_H
 #define OM355_H
@@ -288,6 +289,16 @@ int om_sgm(const float *cost, int64_t B, int32_t D, int32_t H, int32_t W, float 
  * of 3 floats or NULL; mask: device or NULL.  Bit-identical. */
 int om_depth_view(const uint8_t *rgb, int32_t h, int32_t w, const float *gain, const float *bias, const uint8_t *mask,
                   float *grey, uint8_t *rgb8, void *stream);
+
+/* The depth-map file set of densify's passes (densify.py:590-592, :613-614, :639-641).
+ * om_f32_to_f16: score.astype(np.float16) bit for bit (round to nearest even, overflow to inf,
+ * subnormals, NaN keeps its sign and top payload bits); om_f16_to_f32: .astype(np.float32), exact;
+ * om_filter_depth: depth[(f32(score16) < f32(min_ncc)) | low] = NaN in place (low may be NULL).
+ * All arrays device, n elements. */
+int om_f32_to_f16(const float *src, int64_t n, uint16_t *dst, void *stream);
+int om_f16_to_f32(const uint16_t *src, int64_t n, float *dst, void *stream);
+int om_filter_depth(float *depth, const uint16_t *score16, const uint8_t *low, int64_t n, double min_ncc,
+                    void *stream);
 
 /* densify._init_normals (densify.py:481-496): normals (3, H, W) of a depth map, float32 */
 int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream);

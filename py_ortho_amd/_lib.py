@@ -126,6 +126,9 @@ SIGNATURES = {
                                  c_size_t, c_void_p, c_void_p]),
     "om_consistency": (c_int32, [c_void_p, ctypes.POINTER(OmCam), c_int32, ctypes.POINTER(c_void_p),
                                  ctypes.POINTER(OmCam), ctypes.POINTER(c_double), c_double, c_void_p, c_void_p]),
+    "om_f32_to_f16": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "om_f16_to_f32": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "om_filter_depth": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_double, c_void_p]),
     "om_depth_view": (c_int32, [c_void_p, c_int32, c_int32, ctypes.POINTER(ctypes.c_float),
                                 ctypes.POINTER(ctypes.c_float), c_void_p, c_void_p, c_void_p, c_void_p]),
     "om_sgm_workspace_bytes": (c_size_t, [c_int64, c_int32, c_int32, c_int32]),

@@ -79,3 +79,11 @@ def test_used_map_helpers():
     g[3, 5] = 1
     full = fusion.used_full(g, 121, 161, 3)
     assert full.shape == (121, 161) and full.sum() == 1 and full[9, 15]
+
+
+def test_dense_cloud_fields():
+    """formats.DenseCloud has densify.DenseCloud's fields, in order (densify.py:84-90)."""
+    import dataclasses
+
+    from py_ortho_amd import formats
+    assert [f.name for f in dataclasses.fields(formats.DenseCloud)] == ["xyz", "rgb", "views", "cam", "spacing"]
