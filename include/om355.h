@@ -184,6 +184,9 @@ int om_batch_capture(om_batch *b, void *stream);
 int om_batch_replay(om_batch *b, void *stream);
 /* accepted PatchMatch changes per view (synchronises the device) */
 int om_batch_pm_changed(om_batch *b, int64_t *changed);
+/* After a run: the propagation cost evaluations each view performed (caller's view order), as
+ * om_pm_workspace_counts reports them for one view; -1 where the batch runs without the memo. */
+int om_batch_pm_evaluated(om_batch *b, int64_t *evaluated);
 
 /* One step of a continuous-batching server: for every view, hypotheses [i0, i0 + nh) continuing its
  * winner state (views of any size, source count and hypothesis generator; same r and min(top_k, n)),

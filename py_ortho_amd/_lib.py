@@ -111,6 +111,7 @@ SIGNATURES = {
     "om_batch_capture": (c_int32, [c_void_p, c_void_p]),
     "om_batch_replay": (c_int32, [c_void_p, c_void_p]),
     "om_batch_pm_changed": (c_int32, [c_void_p, ctypes.POINTER(c_int64)]),
+    "om_batch_pm_evaluated": (c_int32, [c_void_p, ctypes.POINTER(c_int64)]),
     "om_init_normals": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
     "om_to_lp": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     "om_pm_workspace_counts": (c_int32, [c_void_p, c_int32, c_int32, ctypes.POINTER(c_int64)]),

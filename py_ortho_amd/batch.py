@@ -154,6 +154,12 @@ class Batch:
         check(self._lib.om_batch_pm_changed(self._h, out), "batch pm_changed")
         return [int(out[i]) for i in range(self.n_pm)]
 
+    def pm_evaluated(self) -> List[int]:
+        """Propagation cost evaluations each view performed in the last run (-1: no memo)."""
+        out = (ctypes.c_int64 * max(self.n_pm, 1))()
+        check(self._lib.om_batch_pm_evaluated(self._h, out), "batch pm_evaluated")
+        return [int(out[i]) for i in range(self.n_pm)]
+
 
 def init_normals(depth: torch.Tensor, ray: torch.Tensor) -> torch.Tensor:
     """densify._init_normals (densify.py:481-496) on the device, float32 (3, H, W)."""

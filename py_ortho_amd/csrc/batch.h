@@ -35,12 +35,15 @@ struct om_batch {
     int pm_n = 0, pm_iters = 0, pm_rows = 0, pm_ngroups = 0;
     PMGroup pm_groups[4];
     bool pm_normals = false, pm_prepare = false;
+    bool pm_memo = false;         // memoised propagation (every view small enough for its item list)
     int64_t pm_px = 0;
     void *pm_views = nullptr;     // PMK[pm_n], ordered by source bucket
     int *pm_blk0 = nullptr;       // per group: first block of each of its views (nv + 1 each)
     int *pm_row0 = nullptr;       // first row of each view (pm_n + 1)
     int64_t *pm_px0 = nullptr;    // first pixel of each view (pm_n + 1)
     unsigned long long *pm_changed = nullptr;  // (pm_n,) in the caller's view order
+    std::vector<unsigned long long *> pm_nevals;  // memo path: each view's evaluation count (device)
+    std::vector<char> pm_host;    // host copy of the PMK table (memo path: launched view by view)
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

@@ -685,10 +685,10 @@ __global__ void pm_memo_reset_kernel(const PMK P, int64_t HW, int clear_chg)
         P.chg[i] = 0;
 }
 
-// (1) which costs the pass needs, appended to the item list a wave at a time
-__global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const PassK Q)
+// (1) which costs the pass needs, appended to the item list a wave at a time (every lane of a
+// wave belongs to one view)
+__device__ __forceinline__ void pm_need_px(const PMK &P, const PassK &Q, int64_t t)
 {
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int y, x;
     unsigned mask = 0;
     size_t p = 0;
@@ -728,6 +728,11 @@ __global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const 
             P.items[base + __popcll(m & below)] = (int)(p * 16 + slot);
         base += __popcll(m);
     }
+}
+
+__global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const PassK Q)
+{
+    pm_need_px(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 // (2) one cost per lane.  The grid covers the largest possible list, the item count is known only
@@ -780,9 +785,8 @@ __device__ __forceinline__ void pm_eval_item(const PMK &P, int64_t i)
 }
 
 // (3) the reference's accept sequence over the memoised costs
-__global__ void __launch_bounds__(kPMThreads) pm_decide_kernel(const PMK P, const PassK Q)
+__device__ __forceinline__ void pm_decide_px(const PMK &P, const PassK &Q, int64_t t)
 {
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int y, x;
     if (!colour_pixel(P, Q, t, y, x))
         return;
@@ -813,6 +817,11 @@ __global__ void __launch_bounds__(kPMThreads) pm_decide_kernel(const PMK P, cons
     P.sc[p] = sc;
     P.chg[p] = changed ? 1 : 0;
     count_changes(P, changed);
+}
+
+__global__ void __launch_bounds__(kPMThreads) pm_decide_kernel(const PMK P, const PassK Q)
+{
+    pm_decide_px(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- single-view kernels
@@ -872,6 +881,29 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_batch(const P
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int v = view_of(blk0, nviews, b);
     pm_random_px<SM, 0>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
+}
+
+// item counters of every view; with clear_chg also the evaluation counts and the changed flags of
+// every pixel (px0: first pixel of each view)
+__global__ void pm_memo_reset_batch(const PMK *views, const int64_t *px0, int nviews, int clear_chg)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nviews) {
+        *views[t].nitems = 0;
+        if (clear_chg)
+            *views[t].nevals = 0ull;
+    }
+    if (!clear_chg || t >= px0[nviews])
+        return;
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (px0[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    views[lo].chg[t - px0[lo]] = 0;
 }
 
 __global__ void __launch_bounds__(256) pm_rank_rows_batch(const PMK *views, const int *row0, int nviews,
@@ -1273,7 +1305,11 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
     if (step <= 0 || top_k < 1 || iters < 0)
         return fail(OM_EINVAL, "pm_refine needs step >= 1, top_k >= 1, iters >= 0");
     std::vector<PMK> P(nviews);
+    bool memo = true;   // the memo path lists a view's items as pixel * 16 + slot in an int
+    for (int v = 0; v < nviews; ++v)
+        memo = memo && pm_memo_ok(views[v].H, views[v].W);
     b->pm_changed = (unsigned long long *)b->dev_alloc(sizeof(unsigned long long) * nviews);
+    b->pm_nevals.assign(memo ? nviews : 0, nullptr);
     if (!b->pm_changed)
         return fail(OM_ENOMEM, "batch: counters");
     for (int v = 0; v < nviews; ++v) {
@@ -1282,11 +1318,11 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
             return fail(OM_ENSRC, "view %d: PatchMatch needs 1..16 sources", v);
         if (V.H < 1 || V.W < 1)
             return fail(OM_ESHAPE, "view %d: empty image", v);
-        char *ws = (char *)b->dev_alloc(pm_ws_layout(V.H, V.W, nullptr, nullptr));
+        char *ws = (char *)b->dev_alloc(pm_ws_layout(V.H, V.W, nullptr, nullptr, memo));
         if (!ws)
             return fail(OM_ENOMEM, "batch: PatchMatch workspace");
         PMWs w;
-        pm_ws_layout(V.H, V.W, ws, &w);
+        pm_ws_layout(V.H, V.W, ws, &w, memo);
         PMCam cams[16];
         memset(cams, 0, sizeof cams);
         PMK &Q = P[v];
@@ -1339,6 +1375,13 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         Q.cam = w.cam;
         Q.depth_from = V.depth_from;
         Q.score_from = V.score_from;
+        Q.cc = w.cc;
+        Q.chg = w.chg;
+        Q.items = w.items;
+        Q.nitems = w.nitems;
+        Q.nevals = w.nevals;
+        if (memo)
+            b->pm_nevals[v] = w.nevals;
     }
     // views ordered by source bucket: the propagation / random kernels run once per bucket with
     // the register footprint of that bucket (4, 8, 12 or 16 sources)
@@ -1387,15 +1430,39 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         (rc = hip_check(hipMemcpy(b->pm_px0, px0.data(), sizeof(int64_t) * (nviews + 1), hipMemcpyHostToDevice),
                         "upload")))
         return rc;
+    b->pm_host.assign((const char *)Ps.data(), (const char *)Ps.data() + sizeof(PMK) * nviews);
     b->pm_n = nviews;
     b->pm_iters = iters;
     b->pm_rows = row0[nviews];
     b->pm_px = px0[nviews];
     b->pm_normals = init_normals != 0;
+    b->pm_memo = memo;
     b->pm_prepare = b->pm_normals;
     for (int v = 0; v < nviews; ++v)
         b->pm_prepare = b->pm_prepare || views[v].depth_from || views[v].score_from;
     return OM_OK;
+}
+
+// the memoised propagation of one source bucket, view by view: the views of a bucket together hold
+// more source images and depth maps than the last-level cache, one view's fit; the launches are
+// recorded in the batch's graph, so launching per view costs little
+static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, hipStream_t st)
+{
+    const PMK *host = (const PMK *)b->pm_host.data();
+    for (int i = g.first; i < g.first + g.nv; ++i) {
+        const PMK &P = host[i];
+        const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
+        const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
+        const unsigned ge = (unsigned)(cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD);
+        hipLaunchKernelGGL(pm_need_kernel, dim3(gA), dim3(kPMThreads), 0, st, P, Q);
+        switch (g.sm) {
+        case 4: hipLaunchKernelGGL((pm_eval_kernel<4, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
+        case 8: hipLaunchKernelGGL((pm_eval_kernel<8, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
+        case 12: hipLaunchKernelGGL((pm_eval_kernel<12, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
+        default: hipLaunchKernelGGL((pm_eval_kernel<16, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
+        }
+        hipLaunchKernelGGL(pm_decide_kernel, dim3(gA), dim3(kPMThreads), 0, st, P, Q);
+    }
 }
 
 static void pm_group_launch(const om_batch *b, const PMGroup &g, bool propagate, const PassK &Q, hipStream_t st)
@@ -1433,8 +1500,16 @@ static void pm_batch_passes(om_batch *b, hipStream_t st)
             Q.first = it == 0;
             {
                 ProfScope ps("pm_propagate_batch", st);
-                for (int gi = 0; gi < b->pm_ngroups; ++gi)
-                    pm_group_launch(b, b->pm_groups[gi], true, Q, st);
+                if (b->pm_memo) {
+                    if (pass > 0)
+                        hipLaunchKernelGGL(pm_memo_reset_batch, dim3((unsigned)cdiv(b->pm_n, 64)), dim3(64), 0, st, V,
+                                           b->pm_px0, b->pm_n, 0);
+                    for (int gi = 0; gi < b->pm_ngroups; ++gi)
+                        pm_group_memo(b, b->pm_groups[gi], Q, st);
+                } else {
+                    for (int gi = 0; gi < b->pm_ngroups; ++gi)
+                        pm_group_launch(b, b->pm_groups[gi], true, Q, st);
+                }
             }
             hipLaunchKernelGGL(pm_rank_rows_batch, dim3((unsigned)b->pm_rows), dim3(256), 0, st, V, b->pm_row0,
                                b->pm_n, Q);
@@ -1458,11 +1533,32 @@ int batch_pm_enqueue(om_batch *b, hipStream_t st)
     if (b->pm_prepare)
         hipLaunchKernelGGL(pm_prepare_batch, dim3((unsigned)cdiv(b->pm_px, 256)), dim3(256), 0, st, V, b->pm_px0,
                            b->pm_n, (int)b->pm_normals);
+    if (b->pm_memo)
+        hipLaunchKernelGGL(pm_memo_reset_batch, dim3((unsigned)cdiv(b->pm_px > b->pm_n ? b->pm_px : b->pm_n, 256)),
+                           dim3(256), 0, st, V, b->pm_px0, b->pm_n, 1);
     pm_batch_passes(b, st);
     return launch_check("pm batch kernels");
 }
 
 }  // namespace om
+
+extern "C" int om_batch_pm_evaluated(om_batch *b, int64_t *evaluated)
+{
+    if (!b || !evaluated)
+        return fail(OM_EINVAL, "null argument");
+    if (b->pm_nevals.empty()) {
+        for (int v = 0; v < b->pm_n; ++v)
+            evaluated[v] = -1;
+        return OM_OK;
+    }
+    for (int v = 0; v < b->pm_n; ++v) {
+        unsigned long long h = 0;
+        if (int rc = hip_check(hipMemcpy(&h, b->pm_nevals[v], sizeof h, hipMemcpyDeviceToHost), "copy evaluations"))
+            return rc;
+        evaluated[v] = (int64_t)h;
+    }
+    return OM_OK;
+}
 
 extern "C" int om_init_normals(const float *depth, const float *ray, int32_t H, int32_t W, float *normal, void *stream)
 {
