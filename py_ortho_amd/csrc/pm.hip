@@ -109,6 +109,12 @@ struct PMK {
     uint8_t *chg;
     int *items, *nitems;
     unsigned long long *nevals;  // cost evaluations the memo path performed (all passes of a call)
+    // first-iteration item order: items bucketed by the tile of source 0 their plane's centre
+    // falls in (tiles of 2^sort_shift pixels, sort_tx x sort_ty of them), by their own tile of the
+    // reference image when it falls outside (sort_rx x sort_ry tiles of 2^sort_rshift), and one
+    // bin for items without a plane
+    int *sorted, *bins;
+    int sort_tx, sort_ty, sort_shift, sort_rx, sort_ry, sort_rshift;
 };
 
 // One source view as the cost kernels read it (device memory, uniform loads): the reference's
@@ -185,6 +191,7 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 #endif
 constexpr int kGroup = OM_PM_GROUP;  // sources accumulated together in the cost's sample loop
 constexpr int kPMThreads = 128;
+constexpr int kSortBins = 65536;  // most tiles of source 0 the first-iteration item order uses
 
 // per-thread LDS slots for the S source scores of one cost evaluation
 __device__ __forceinline__ double *pm_ncc_scratch()
@@ -735,6 +742,102 @@ __global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const 
     pm_need_px(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
 
+// First iteration: the planes are still the random start, so a wave of neighbouring pixels samples
+// the sources far apart.  The items are put in the order of the tile of source 0 that each plane's
+// centre projects to (a counting sort; the order inside a bin is whatever the atomics give).
+// Each item writes its own memo slot, so the order changes no result.
+__device__ __forceinline__ int pm_item_bin(const PMK &P, int it)
+{
+    const size_t p = (size_t)(it >> 4);
+    const int slot = it & 15;
+    const int y = (int)(p / P.W), x = (int)(p - (size_t)y * P.W);
+    const size_t HW = (size_t)P.H * P.W;
+    const int n0 = P.sort_tx * P.sort_ty, none = n0 + P.sort_rx * P.sort_ry;
+    double dp;
+    if (slot == 8) {
+        const float d0 = P.depth[p];
+        if (!isfinite(d0))
+            return none;
+        dp = d0;
+    } else {
+        double b0, b1, b2;
+        if (!pm_candidate(P, y, x, slot, dp, b0, b1, b2))
+            return none;
+    }
+    const double X = P.cr0 + dp * P.ray[p], Y = P.cr1 + dp * P.ray[HW + p], Z = P.cr2 + dp * P.ray[2 * HW + p];
+    double u, v, z;
+    const double ts = (double)(1 << P.sort_shift);
+    if (pm_proj(P, 0, X, Y, Z, u, v, z) && u >= 0.0 && v >= 0.0 && u < ts * P.sort_tx && v < ts * P.sort_ty)
+        return (int)(v / ts) * P.sort_tx + (int)(u / ts);
+    return n0 + (y >> P.sort_rshift) * P.sort_rx + (x >> P.sort_rshift);   // outside source 0
+}
+
+__global__ void pm_sort_zero_kernel(int *bins, int nb)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x)
+        bins[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) pm_sort_hist_kernel(const PMK P)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *P.nitems)
+        atomicAdd(&P.bins[pm_item_bin(P, P.items[i])], 1);
+}
+
+// exclusive scan of the bin counts in place (one workgroup)
+__global__ void __launch_bounds__(1024) pm_sort_scan_kernel(int *bins, int nb)
+{
+    __shared__ int wtot[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + tid;
+        const int x = i < nb ? bins[i] : 0;
+        int y = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int z = __shfl_up(y, o, 64);
+            if (lane >= o)
+                y += z;
+        }
+        if (lane == 63)
+            wtot[wave] = y;
+        __syncthreads();
+        int before = 0, all = 0;
+        for (int w = 0; w < 16; ++w) {
+            before += w < wave ? wtot[w] : 0;
+            all += wtot[w];
+        }
+        if (i < nb)
+            bins[i] = carry + before + y - x;
+        carry += all;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) pm_sort_scatter_kernel(const PMK P)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *P.nitems) {
+        const int it = P.items[i];
+        P.sorted[atomicAdd(&P.bins[pm_item_bin(P, it)], 1)] = it;
+    }
+}
+
+// enqueue the sort; returns the view with its item list replaced by the sorted one
+static PMK pm_sort_items(const PMK &P, int64_t ncol, hipStream_t st)
+{
+    const int nb = P.sort_tx * P.sort_ty + P.sort_rx * P.sort_ry + 1;
+    const unsigned g = (unsigned)cdiv(9 * ncol, 256);
+    hipLaunchKernelGGL(pm_sort_zero_kernel, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, st, P.bins, nb);
+    hipLaunchKernelGGL(pm_sort_hist_kernel, dim3(g), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(pm_sort_scan_kernel, dim3(1), dim3(1024), 0, st, P.bins, nb);
+    hipLaunchKernelGGL(pm_sort_scatter_kernel, dim3(g), dim3(256), 0, st, P);
+    PMK Q = P;
+    Q.items = P.sorted;
+    return Q;
+}
+
 // (2) one cost per lane.  The grid covers the largest possible list, the item count is known only
 // on the device: the blocks that have items are mapped so that each XCD (hardware dispatch is
 // round-robin over the 8 XCDs) gets an equal, contiguous share -- equal work, and neighbouring
@@ -934,6 +1037,7 @@ struct PMWs {
     uint32_t *st;  // 2 states
     unsigned long long *changed;
     PMCam *cam;    // 16 source tables
+    int *sorted, *bins;  // first-iteration item order
     double *cc;    // memo (single view): (H, W, 9, 2)
     uint8_t *chg;  // (H, W)
     int *items, *nitems;
@@ -942,6 +1046,24 @@ struct PMWs {
 
 // the memo path lists items as pixel * 16 + slot in an int
 static bool pm_memo_ok(int H, int W) { return (int64_t)H * W < (1LL << 27); }
+
+// tiles of source 0 for the first-iteration item order: 16 pixels, larger for images beyond
+// kSortBins tiles
+static void pm_sort_tiles(PMK &P, int iw0, int ih0)
+{
+    int sh = 4;
+    while (cdiv(iw0, 1 << sh) * cdiv(ih0, 1 << sh) > kSortBins)
+        ++sh;
+    P.sort_shift = sh;
+    P.sort_tx = (int)cdiv(iw0, 1 << sh);
+    P.sort_ty = (int)cdiv(ih0, 1 << sh);
+    int rs = 4;
+    while (cdiv(P.W, 1 << rs) * cdiv(P.H, 1 << rs) > kSortBins)
+        ++rs;
+    P.sort_rshift = rs;
+    P.sort_rx = (int)cdiv(P.W, 1 << rs);
+    P.sort_ry = (int)cdiv(P.H, 1 << rs);
+}
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -956,7 +1078,7 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
     };
     char *b = take(HW * 8), *s = take(HW * 8), *rk = take(HW * 4), *rc = take((size_t)H * 4),
          *ro = take((size_t)H * 4), *st = take(16), *ch = take(8), *cm = take(sizeof(PMCam) * 16);
-    char *cc = nullptr, *cg = nullptr, *it = nullptr, *ni = nullptr, *ne = nullptr;
+    char *cc = nullptr, *cg = nullptr, *it = nullptr, *ni = nullptr, *ne = nullptr, *so = nullptr, *bn = nullptr;
     if (memo) {
         const size_t ncol = (size_t)H * ((W + 1) / 2);
         cc = take(HW * 18 * sizeof(double));
@@ -964,8 +1086,12 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
         it = take(ncol * 9 * sizeof(int));
         ni = take(sizeof(int));
         ne = take(sizeof(unsigned long long));
+        so = take(ncol * 9 * sizeof(int));
+        bn = take((size_t)(2 * kSortBins + 1) * sizeof(int));
     }
     if (w) {
+        w->sorted = (int *)so;
+        w->bins = (int *)bn;
         w->cc = (double *)cc;
         w->chg = (uint8_t *)cg;
         w->items = (int *)it;
@@ -996,6 +1122,9 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
     P.items = w.items;
     P.nitems = w.nitems;
     P.nevals = w.nevals;
+    P.sorted = w.sorted;
+    P.bins = w.bins;
+    pm_sort_tiles(P, A.cam[0].iw, A.cam[0].ih);
     const int64_t HW = (int64_t)P.H * P.W;
     // one block per 128 possible items, rounded up to whole rounds over the XCDs
     const int64_t eval_grid = cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD;
@@ -1016,10 +1145,11 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
                 if (pass > 0)
                     hipLaunchKernelGGL(pm_memo_reset_kernel, dim3(1), dim3(64), 0, stream, P, HW, 0);
                 hipLaunchKernelGGL(pm_need_kernel, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                const PMK Pe = Q.first ? pm_sort_items(P, ncol, stream) : P;
                 {
                     ProfScope pe("pm_eval", stream);
                     hipLaunchKernelGGL((pm_eval_kernel<SM, FMT>), dim3((unsigned)eval_grid), dim3(kPMThreads), 0,
-                                       stream, P);
+                                       stream, Pe);
                 }
                 hipLaunchKernelGGL(pm_decide_kernel, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
             } else {
@@ -1380,6 +1510,9 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         Q.items = w.items;
         Q.nitems = w.nitems;
         Q.nevals = w.nevals;
+        Q.sorted = w.sorted;
+        Q.bins = w.bins;
+        pm_sort_tiles(Q, V.iw[0], V.ih[0]);
         if (memo)
             b->pm_nevals[v] = w.nevals;
     }
@@ -1455,11 +1588,12 @@ static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, h
         const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
         const unsigned ge = (unsigned)(cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD);
         hipLaunchKernelGGL(pm_need_kernel, dim3(gA), dim3(kPMThreads), 0, st, P, Q);
+        const PMK Pe = Q.first ? pm_sort_items(P, ncol, st) : P;
         switch (g.sm) {
-        case 4: hipLaunchKernelGGL((pm_eval_kernel<4, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
-        case 8: hipLaunchKernelGGL((pm_eval_kernel<8, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
-        case 12: hipLaunchKernelGGL((pm_eval_kernel<12, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
-        default: hipLaunchKernelGGL((pm_eval_kernel<16, 0>), dim3(ge), dim3(kPMThreads), 0, st, P); break;
+        case 4: hipLaunchKernelGGL((pm_eval_kernel<4, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
+        case 8: hipLaunchKernelGGL((pm_eval_kernel<8, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
+        case 12: hipLaunchKernelGGL((pm_eval_kernel<12, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
+        default: hipLaunchKernelGGL((pm_eval_kernel<16, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
         }
         hipLaunchKernelGGL(pm_decide_kernel, dim3(gA), dim3(kPMThreads), 0, st, P, Q);
     }
