@@ -110,11 +110,12 @@ struct PMK {
     int *items, *nitems;
     unsigned long long *nevals;  // cost evaluations the memo path performed (all passes of a call)
     // first-iteration item order: items bucketed by the tile of source 0 their plane's centre
-    // falls in (tiles of 2^sort_shift pixels, sort_tx x sort_ty of them), by their own tile of the
-    // reference image when it falls outside (sort_rx x sort_ry tiles of 2^sort_rshift), and one
-    // bin for items without a plane
+    // falls in (tiles of 2^sort_shift pixels, sort_tx x sort_ty of them), or by their own tile of
+    // the reference image (sort_rx x sort_ry tiles of 2^sort_rshift) when it falls outside or the
+    // item has no plane
     int *sorted, *bins;
     int sort_tx, sort_ty, sort_shift, sort_rx, sort_ry, sort_rshift;
+    const int *sort_use;  // set for an evaluation after a sort: nonzero = read the sorted list
 };
 
 // One source view as the cost kernels read it (device memory, uniform loads): the reference's
@@ -752,24 +753,25 @@ __device__ __forceinline__ int pm_item_bin(const PMK &P, int it)
     const int slot = it & 15;
     const int y = (int)(p / P.W), x = (int)(p - (size_t)y * P.W);
     const size_t HW = (size_t)P.H * P.W;
-    const int n0 = P.sort_tx * P.sort_ty, none = n0 + P.sort_rx * P.sort_ry;
+    const int n0 = P.sort_tx * P.sort_ty;
+    const int own = n0 + (y >> P.sort_rshift) * P.sort_rx + (x >> P.sort_rshift);   // reference tile
     double dp;
     if (slot == 8) {
         const float d0 = P.depth[p];
         if (!isfinite(d0))
-            return none;
+            return own;
         dp = d0;
     } else {
         double b0, b1, b2;
         if (!pm_candidate(P, y, x, slot, dp, b0, b1, b2))
-            return none;
+            return own;
     }
     const double X = P.cr0 + dp * P.ray[p], Y = P.cr1 + dp * P.ray[HW + p], Z = P.cr2 + dp * P.ray[2 * HW + p];
     double u, v, z;
     const double ts = (double)(1 << P.sort_shift);
     if (pm_proj(P, 0, X, Y, Z, u, v, z) && u >= 0.0 && v >= 0.0 && u < ts * P.sort_tx && v < ts * P.sort_ty)
         return (int)(v / ts) * P.sort_tx + (int)(u / ts);
-    return n0 + (y >> P.sort_rshift) * P.sort_rx + (x >> P.sort_rshift);   // outside source 0
+    return own;   // outside source 0
 }
 
 __global__ void pm_sort_zero_kernel(int *bins, int nb)
@@ -778,10 +780,46 @@ __global__ void pm_sort_zero_kernel(int *bins, int nb)
         bins[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) pm_sort_hist_kernel(const PMK P)
+// Whether to sort: a PatchMatch started from a sweep already has coherent planes (neighbouring
+// items share a bin and sorting gains nothing); the random start scatters them.  One workgroup
+// looks at the first 64 x 64 items: bins[nb + 1] = 1 when on average fewer than 4 items of a
+// 64-item run share a bin.
+__global__ void __launch_bounds__(256) pm_sort_probe_kernel(const PMK P, int nb)
+{
+    __shared__ int s_distinct, s_seen;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t n = *P.nitems;
+    if (threadIdx.x == 0) {
+        s_distinct = 0;
+        s_seen = 0;
+    }
+    __syncthreads();
+    for (int run = wave; run < 64; run += 4) {
+        const int64_t i = (int64_t)run * 64 + lane;
+        const bool on = i < n;
+        const int bin = on ? pm_item_bin(P, P.items[i]) : 0;
+        unsigned long long todo = __ballot(on);
+        const int seen = __popcll(todo);
+        int distinct = 0;
+        while (todo) {
+            const int b = __shfl(bin, __ffsll((long long)todo) - 1, 64);
+            todo &= ~__ballot(on && bin == b);
+            ++distinct;
+        }
+        if (lane == 0) {
+            atomicAdd(&s_distinct, distinct);
+            atomicAdd(&s_seen, seen);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        P.bins[nb + 1] = s_seen > 0 && 4 * s_distinct >= s_seen;
+}
+
+__global__ void __launch_bounds__(256) pm_sort_hist_kernel(const PMK P, int nb)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *P.nitems)
+    if (P.bins[nb + 1] && i < *P.nitems)
         atomicAdd(&P.bins[pm_item_bin(P, P.items[i])], 1);
 }
 
@@ -790,6 +828,8 @@ __global__ void __launch_bounds__(1024) pm_sort_scan_kernel(int *bins, int nb)
 {
     __shared__ int wtot[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (!bins[nb + 1])
+        return;
     int carry = 0;
     for (int base = 0; base < nb; base += 1024) {
         const int i = base + tid;
@@ -815,26 +855,27 @@ __global__ void __launch_bounds__(1024) pm_sort_scan_kernel(int *bins, int nb)
     }
 }
 
-__global__ void __launch_bounds__(256) pm_sort_scatter_kernel(const PMK P)
+__global__ void __launch_bounds__(256) pm_sort_scatter_kernel(const PMK P, int nb)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *P.nitems) {
+    if (P.bins[nb + 1] && i < *P.nitems) {
         const int it = P.items[i];
         P.sorted[atomicAdd(&P.bins[pm_item_bin(P, it)], 1)] = it;
     }
 }
 
-// enqueue the sort; returns the view with its item list replaced by the sorted one
+// enqueue the sort; returns the view whose evaluation reads the sorted list when the sort ran
 static PMK pm_sort_items(const PMK &P, int64_t ncol, hipStream_t st)
 {
-    const int nb = P.sort_tx * P.sort_ty + P.sort_rx * P.sort_ry + 1;
+    const int nb = P.sort_tx * P.sort_ty + P.sort_rx * P.sort_ry;
     const unsigned g = (unsigned)cdiv(9 * ncol, 256);
-    hipLaunchKernelGGL(pm_sort_zero_kernel, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, st, P.bins, nb);
-    hipLaunchKernelGGL(pm_sort_hist_kernel, dim3(g), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(pm_sort_zero_kernel, dim3((unsigned)cdiv(nb + 2, 256)), dim3(256), 0, st, P.bins, nb + 2);
+    hipLaunchKernelGGL(pm_sort_probe_kernel, dim3(1), dim3(256), 0, st, P, nb);
+    hipLaunchKernelGGL(pm_sort_hist_kernel, dim3(g), dim3(256), 0, st, P, nb);
     hipLaunchKernelGGL(pm_sort_scan_kernel, dim3(1), dim3(1024), 0, st, P.bins, nb);
-    hipLaunchKernelGGL(pm_sort_scatter_kernel, dim3(g), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(pm_sort_scatter_kernel, dim3(g), dim3(256), 0, st, P, nb);
     PMK Q = P;
-    Q.items = P.sorted;
+    Q.sort_use = P.bins + nb + 1;
     return Q;
 }
 
@@ -843,7 +884,7 @@ static PMK pm_sort_items(const PMK &P, int64_t ncol, hipStream_t st)
 // round-robin over the 8 XCDs) gets an equal, contiguous share -- equal work, and neighbouring
 // items in one L2 -- and the surplus blocks of every XCD exit at once.
 template <int SM, int FMT>
-__device__ __forceinline__ void pm_eval_item(const PMK &P, int64_t i);
+__device__ __forceinline__ void pm_eval_item(const PMK &P, const int *items, int64_t i);
 
 template <int SM, int FMT>
 __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_eval_kernel(const PMK P)
@@ -856,13 +897,13 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_eval_kernel(const PM
         return;
     const int64_t i = ((int64_t)xcd * per + j) * blockDim.x + threadIdx.x;
     if (i < n)
-        pm_eval_item<SM, FMT>(P, i);
+        pm_eval_item<SM, FMT>(P, (P.sort_use && *P.sort_use) ? P.sorted : P.items, i);
 }
 
 template <int SM, int FMT>
-__device__ __forceinline__ void pm_eval_item(const PMK &P, int64_t i)
+__device__ __forceinline__ void pm_eval_item(const PMK &P, const int *items, int64_t i)
 {
-    const int it = P.items[i];
+    const int it = items[i];
     const size_t p = (size_t)(it >> 4);
     const int slot = it & 15;
     const int y = (int)(p / P.W), x = (int)(p - (size_t)y * P.W);
@@ -1087,7 +1128,7 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
         ni = take(sizeof(int));
         ne = take(sizeof(unsigned long long));
         so = take(ncol * 9 * sizeof(int));
-        bn = take((size_t)(2 * kSortBins + 1) * sizeof(int));
+        bn = take((size_t)(2 * kSortBins + 2) * sizeof(int));
     }
     if (w) {
         w->sorted = (int *)so;
