@@ -227,8 +227,10 @@ def cpu_baseline_pm(c, torch, args):
                         c["seed"])
         return time.perf_counter() - t
 
-    probe = crop(16, 1, 1)                         # seconds per 256 pixel-iterations, one core
-    per_px_it = probe / 256.0
+    # marginal seconds per pixel-iteration on one core (the difference of two probes cancels the
+    # call's fixed cost, which made a single small probe pick too small a crop)
+    t1, t2 = crop(24, 1, 1), crop(48, 1, 1)
+    per_px_it = (t2 - t1) / (48 * 48 - 24 * 24) if t2 > t1 else t2 / (48 * 48)
     target = 1.5                                   # seconds of work per core (bounded sample)
     side = int(max(16, min(256, math.sqrt(target / (per_px_it * c["iters"])))))
     wall = crop(side, c["iters"], cores)

@@ -747,7 +747,7 @@ __global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const 
 // the sources far apart.  The items are put in the order of the tile of source 0 that each plane's
 // centre projects to (a counting sort; the order inside a bin is whatever the atomics give).
 // Each item writes its own memo slot, so the order changes no result.
-__device__ __forceinline__ int pm_item_bin(const PMK &P, int it)
+__device__ __forceinline__ int pm_item_bin(const PMK &P, int it, bool *has_plane = nullptr)
 {
     const size_t p = (size_t)(it >> 4);
     const int slot = it & 15;
@@ -756,6 +756,8 @@ __device__ __forceinline__ int pm_item_bin(const PMK &P, int it)
     const int n0 = P.sort_tx * P.sort_ty;
     const int own = n0 + (y >> P.sort_rshift) * P.sort_rx + (x >> P.sort_rshift);   // reference tile
     double dp;
+    if (has_plane)
+        *has_plane = false;
     if (slot == 8) {
         const float d0 = P.depth[p];
         if (!isfinite(d0))
@@ -766,6 +768,8 @@ __device__ __forceinline__ int pm_item_bin(const PMK &P, int it)
         if (!pm_candidate(P, y, x, slot, dp, b0, b1, b2))
             return own;
     }
+    if (has_plane)
+        *has_plane = true;
     const double X = P.cr0 + dp * P.ray[p], Y = P.cr1 + dp * P.ray[HW + p], Z = P.cr2 + dp * P.ray[2 * HW + p];
     double u, v, z;
     const double ts = (double)(1 << P.sort_shift);
@@ -782,8 +786,8 @@ __global__ void pm_sort_zero_kernel(int *bins, int nb)
 
 // Whether to sort: a PatchMatch started from a sweep already has coherent planes (neighbouring
 // items share a bin and sorting gains nothing); the random start scatters them.  One workgroup
-// looks at the first 64 x 64 items: bins[nb + 1] = 1 when on average fewer than 4 items of a
-// 64-item run share a bin.
+// looks at 64 runs of 64 consecutive items spread over the list, counting only items that have
+// a plane: bins[nb + 1] = 1 when on average fewer than 4 of them share a bin within a run.
 __global__ void __launch_bounds__(256) pm_sort_probe_kernel(const PMK P, int nb)
 {
     __shared__ int s_distinct, s_seen;
@@ -795,9 +799,10 @@ __global__ void __launch_bounds__(256) pm_sort_probe_kernel(const PMK P, int nb)
     }
     __syncthreads();
     for (int run = wave; run < 64; run += 4) {
-        const int64_t i = (int64_t)run * 64 + lane;
-        const bool on = i < n;
-        const int bin = on ? pm_item_bin(P, P.items[i]) : 0;
+        const int64_t i = (n / 64 / 64) * 64 * run + lane;   // runs spread evenly over the list
+        bool plane = false;
+        const int bin = i < n ? pm_item_bin(P, P.items[i], &plane) : 0;
+        const bool on = plane;
         unsigned long long todo = __ballot(on);
         const int seen = __popcll(todo);
         int distinct = 0;
