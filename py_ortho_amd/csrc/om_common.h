@@ -76,6 +76,47 @@ __device__ __forceinline__ const T &uniform_ref(const T *p)
     return *(const T *)as_const(p);
 }
 
+// xc / zc and yc / zc for zc > 1e-6 (or NaN), each bit-identical to IEEE division.  When both
+// numerators lie in [2^-900, 2^100] in magnitude and zc <= 2^100, the two quotients share one
+// reciprocal of zc and run the steps of the compiler's own f64 division (v_rcp_f64, two Newton
+// steps, quotient, FMA residual, FMA correction): in that range its operand scaling
+// (v_div_scale_f64) and special-case fix-up (v_div_fixup_f64) return their inputs unchanged, so
+// the bits agree.  Zeros and tiny or huge operands take plain division; a NaN operand gives a NaN
+// quotient on either path.
+__device__ __forceinline__ void div2_shared(double xc, double yc, double zc, double &nx, double &ny)
+{
+    const double lo = fmin(fabs(xc), fabs(yc)), hi = fmax(fabs(xc), fabs(yc));
+    if (lo >= 0x1p-900 && hi <= 0x1p100 && zc <= 0x1p100) {
+        double r = __builtin_amdgcn_rcp(zc);
+        r = fma(r, fma(-zc, r, 1.0), r);
+        r = fma(r, fma(-zc, r, 1.0), r);
+        const double qx = xc * r, qy = yc * r;
+        nx = fma(fma(-zc, qx, xc), r, qx);
+        ny = fma(fma(-zc, qy, yc), r, qy);
+    } else {
+        nx = xc / zc;
+        ny = yc / zc;
+    }
+}
+
+// Bilinear tap of a float32 image at 0 <= u <= w - 1.001, 0 <= v <= h - 1.001, in FP64 in the
+// order (a00 (1 - fx) + a01 fx) (1 - fy) + (a10 (1 - fx) + a11 fx) fy with x0 = (int)u and
+// fx = u - x0.  That difference is exact and equals u - floor(u), which v_fract_f64 returns; the
+// two rows are addressed by 32-bit byte offsets from the image base (callers check that images
+// hold fewer than 2^30 pixels).
+__device__ __forceinline__ double bil_f64(const float *im, int w, double u, double v)
+{
+    const uint32_t x0 = (uint32_t)(int)u, y0 = (uint32_t)(int)v;
+    const double fx = __builtin_amdgcn_fract(u), fy = __builtin_amdgcn_fract(v);
+    const uint32_t o0 = (__umul24(y0, (uint32_t)w) + x0) << 2, o1 = o0 + ((uint32_t)w << 2);
+    const float *r0 = (const float *)((const char *)im + o0), *r1 = (const float *)((const char *)im + o1);
+    const double a00 = r0[0], a01 = r0[1], a10 = r1[0], a11 = r1[1];
+    return (a00 * (1.0 - fx) + a01 * fx) * (1.0 - fy) + (a10 * (1.0 - fx) + a11 * fx) * fy;
+}
+
+// most pixels an image read by bil_f64 may hold (32-bit byte offsets)
+constexpr int64_t kMaxTapPixels = int64_t(1) << 30;
+
 struct ProfScope {
     const char *name = nullptr;
     hipStream_t st = nullptr;

@@ -149,16 +149,17 @@ __device__ __forceinline__ bool pm_proj(const PMK &P, int s, double X, double Y,
     const double zc = p[6] * dx + p[7] * dy + p[8] * dz;
     if (zc <= 1e-6)
         return false;
-    const double nx = xc / zc, ny = yc / zc;
-    double dd;
+    double nx, ny;
+    div2_shared(xc, yc, zc, nx, ny);
     if (P.pinhole) {
-        dd = p[12];  // f * (1 + r2 * 0) == f for finite r2
+        u = p[12] * nx + p[16];  // f * (1 + r2 * 0) == f for finite r2
+        v = p[12] * ny + p[17];
     } else {
         const double q = nx * nx + ny * ny;
-        dd = p[12] * (1.0 + q * (p[13] + q * (p[14] + q * p[15])));
+        const double dd = p[12] * (1.0 + q * (p[13] + q * (p[14] + q * p[15])));
+        u = dd * nx + p[16];
+        v = dd * ny + p[17];
     }
-    u = dd * nx + p[16];
-    v = dd * ny + p[17];
     z = zc;
     return true;
 }
@@ -168,11 +169,7 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 {
     if (!(u >= 0.0 && v >= 0.0 && u <= wl && v <= hl))
         return NAN;
-    const int x0 = (int)u, y0 = (int)v;
-    const double fx = u - (double)x0, fy = v - (double)y0;
-    const float *p = im + (size_t)y0 * w + x0;
-    return ((double)p[0] * (1.0 - fx) + (double)p[1] * fx) * (1.0 - fy) +
-           ((double)p[w] * (1.0 - fx) + (double)p[w + 1] * fx) * fy;
+    return bil_f64(im, w, u, v);
 }
 
 // waves per SIMD the PatchMatch cost kernels are compiled for (0: the compiler's choice).  4 waves
@@ -1292,6 +1289,8 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
     for (int s = 0; s < S; ++s) {
         if (ih[s] < 1 || iw[s] < 1)
             return fail(OM_ESHAPE, "source %d has an empty image", s);
+        if ((int64_t)ih[s] * iw[s] >= kMaxTapPixels)
+            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more are not supported", s);
         PMCam &c = A.cam[s];
         c.img = (const float *)imgs[s];
         c.dep = deps[s];
@@ -1507,6 +1506,8 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         for (int s = 0; s < V.S; ++s) {
             if (V.ih[s] < 1 || V.iw[s] < 1)
                 return fail(OM_ESHAPE, "view %d source %d has an empty image", v, s);
+            if ((int64_t)V.ih[s] * V.iw[s] >= kMaxTapPixels)
+                return fail(OM_ESHAPE, "view %d source %d: images of 2^30 pixels or more are not supported", v, s);
             cams[s].img = V.imgs[s];
             cams[s].dep = V.deps[s];
             cams[s].ih = V.ih[s];

@@ -235,22 +235,19 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
                     const double Z = g[11] + d * ((g[6] * rx + g[7] * ry) + g[8] * rz);
                     if (!(Z <= 1e-6)) {
-                        const double nx = X / Z, ny = Y / Z;
-                        double dd;
+                        double nx, ny, u, v;
+                        div2_shared(X, Y, Z, nx, ny);
                         if (P.pinhole) {
-                            dd = g[12];  // f * (1 + r2 * 0) == f for finite r2
+                            u = g[12] * nx + g[16];  // f * (1 + r2 * 0) == f for finite r2
+                            v = g[12] * ny + g[17];
                         } else {
                             const double rr = nx * nx + ny * ny;
-                            dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
+                            const double dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
+                            u = dd * nx + g[16];
+                            v = dd * ny + g[17];
                         }
-                        const double u = dd * nx + g[16], v = dd * ny + g[17];
                         if (u >= 0.0 && v >= 0.0 && u <= wl && v <= hl) {
-                            const int x0 = (int)u, y0 = (int)v;
-                            const double fx = u - (double)x0, fy = v - (double)y0;
-                            const float *pp = img + (size_t)y0 * sw + x0;
-                            const double a00 = pp[0], a01 = pp[1], a10 = pp[sw], a11 = pp[sw + 1];
-                            const double val =
-                                (a00 * (1.0 - fx) + a01 * fx) * (1.0 - fy) + (a10 * (1.0 - fx) + a11 * fx) * fy;
+                            const double val = bil_f64(img, sw, u, v);
                             if (isfinite(val))
                                 J = (float)val;
                         }
@@ -610,6 +607,8 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
     P.pinhole = 1;
     for (int s = 0; s < d->n_src; ++s) {
         const om_src &o = d->src[s];
+        if ((int64_t)o.h * o.w >= kMaxTapPixels)
+            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more are not supported", s);
         P.src[s].img = o.img;
         P.src[s].h = o.h;
         P.src[s].w = o.w;
