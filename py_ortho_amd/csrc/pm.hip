@@ -320,6 +320,20 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
         pm_ncc_lp<FMT>(P, y, x, num, n0, n1, n2, ncc_l);
     } else {
     int nsamp = 0;
+    // The window samples inside the image (the reference's loops skip the others,
+    // _dense.pyx:487-492): rows a = -r + ka st for ka in [ka0, ka1], columns likewise, walked row
+    // by row as one sequence of ns samples.
+    const int nw = 2 * r / st;
+    const int ka0 = y < r ? (r - y + st - 1) / st : 0, ka1 = min(nw, (H - 1 - y + r) / st);
+    const int kb0 = x < r ? (r - x + st - 1) / st : 0, kb1 = min(nw, (W - 1 - x + r) / st);
+    const int ns = (ka1 >= ka0 && kb1 >= kb0) ? (ka1 - ka0 + 1) * (kb1 - kb0 + 1) : 0;
+    // byte offset of window sample (ka, kb) in the reference image and in each ray plane (images
+    // of fewer than 2^30 pixels)
+    const char *refb = (const char *)P.ref, *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
+               *rayz = (const char *)(P.ray + 2 * HW);
+    auto sample_off = [&](int ka, int kb) -> uint32_t {
+        return (__umul24((uint32_t)(y - r + ka * st), (uint32_t)W) + (uint32_t)(x - r + kb * st)) << 2;
+    };
     // Sources in groups of kGroup, the group loop kept rolled: per-sample geometry is recomputed
     // once per group, only kGroup sets of accumulators and one group's camera parameters are live.
     // Each source still sums its samples in window order, so every sum is the reference's.
@@ -333,41 +347,53 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
             cnt[u] = 0;
         }
         nsamp = 0;
-        for (int a = -r; a <= r; a += st) {
-            const int yy = y + a;
-            if (yy < 0 || yy >= H)
+        // Each step loads the next sample's reference value and ray before it works on its own,
+        // so their latency overlaps the current sample's projections.
+        int ka = ka0, kb = kb0;
+        uint32_t off = ns > 0 ? sample_off(ka, kb) : 0u;
+        float In = 0.f, qxn = 0.f, qyn = 0.f, qzn = 0.f;
+        if (ns > 0) {
+            In = *(const float *)(refb + off);
+            qxn = *(const float *)(rayx + off);
+            qyn = *(const float *)(rayy + off);
+            qzn = *(const float *)(rayz + off);
+        }
+        for (int j = 0; j < ns; ++j) {
+            const double I = In, qx = qxn, qy = qyn, qz = qzn;
+            if (j + 1 < ns) {
+                if (++kb > kb1) {
+                    kb = kb0;
+                    ++ka;
+                }
+                off = sample_off(ka, kb);
+            }
+            In = *(const float *)(refb + off);
+            qxn = *(const float *)(rayx + off);
+            qyn = *(const float *)(rayy + off);
+            qzn = *(const float *)(rayz + off);
+            if (!isfinite(I))
                 continue;
-            for (int b = -r; b <= r; b += st) {
-                const int xx = x + b;
-                if (xx < 0 || xx >= W)
-                    continue;
-                const size_t q = (size_t)yy * W + xx;
-                const double I = P.ref[q];
-                if (!isfinite(I))
-                    continue;
-                const double qx = P.ray[q], qy = P.ray[HW + q], qz = P.ray[2 * HW + q];
-                const double den = n0 * qx + n1 * qy + n2 * qz;
-                if (den >= -1e-9)
-                    continue;
-                const double t = num / den;
-                const double X = P.cr0 + t * qx, Y = P.cr1 + t * qy, Z = P.cr2 + t * qz;
-                nsamp += 1;
+            const double den = n0 * qx + n1 * qy + n2 * qz;
+            if (den >= -1e-9)
+                continue;
+            const double t = num / den;
+            const double X = P.cr0 + t * qx, Y = P.cr1 + t * qy, Z = P.cr2 + t * qz;
+            nsamp += 1;
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int s = g0 + u;
-                    if (s < S) {
-                        double pu, pv, pz;
-                        if (pm_proj(P, s, X, Y, Z, pu, pv, pz)) {
-                            const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
-                            const double J = pm_bil(cs.img, cs.iw, cs.wl, cs.hl, pu, pv);
-                            if (isfinite(J)) {
-                                sI[u] += I;
-                                sJ[u] += J;
-                                sII[u] += I * I;
-                                sJJ[u] += J * J;
-                                sIJ[u] += I * J;
-                                cnt[u] += 1;
-                            }
+            for (int u = 0; u < kGroup; ++u) {
+                const int s = g0 + u;
+                if (s < S) {
+                    double pu, pv, pz;
+                    if (pm_proj(P, s, X, Y, Z, pu, pv, pz)) {
+                        const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
+                        const double J = pm_bil(cs.img, cs.iw, cs.wl, cs.hl, pu, pv);
+                        if (isfinite(J)) {
+                            sI[u] += I;
+                            sJ[u] += J;
+                            sII[u] += I * I;
+                            sJJ[u] += J * J;
+                            sIJ[u] += I * J;
+                            cnt[u] += 1;
                         }
                     }
                 }
@@ -1248,6 +1274,8 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
         return fail(OM_EINVAL, "null host array");
     if ((int64_t)H * W == 0 || iters <= 0)
         return OM_OK;
+    if ((int64_t)H * W >= kMaxTapPixels)
+        return fail(OM_ESHAPE, "pm_refine: reference images of 2^30 pixels or more are not supported");
     const bool memo = pm_memo_ok(H, W);
     const size_t need = pm_ws_layout(H, W, nullptr, nullptr, memo);
     if (!workspace || workspace_bytes < need)
@@ -1493,6 +1521,8 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
             return fail(OM_ENSRC, "view %d: PatchMatch needs 1..16 sources", v);
         if (V.H < 1 || V.W < 1)
             return fail(OM_ESHAPE, "view %d: empty image", v);
+        if ((int64_t)V.H * V.W >= kMaxTapPixels)
+            return fail(OM_ESHAPE, "view %d: reference images of 2^30 pixels or more are not supported", v);
         char *ws = (char *)b->dev_alloc(pm_ws_layout(V.H, V.W, nullptr, nullptr, memo));
         if (!ws)
             return fail(OM_ENOMEM, "batch: PatchMatch workspace");
