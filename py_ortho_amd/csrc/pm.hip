@@ -208,6 +208,66 @@ __device__ __forceinline__ float pm_ld16(const uint16_t *p)
     return (float)__builtin_bit_cast(_Float16, v);
 }
 
+// The window samples of pixel (y, x) that lie inside the image (the reference's loops skip the
+// others, _dense.pyx:480-491): rows a = -r + ka st for ka in [ka0, ka1], columns likewise, walked
+// row by row as one sequence of ns samples.  Each step hands out the current sample's reference
+// value and ray and loads the next sample's, so their latency overlaps the current sample's work;
+// loads use 32-bit byte offsets from the image and ray-plane bases (images of fewer than 2^30
+// pixels).
+struct PMWindow {
+    int ns, ka0, kb0, kb1, ka, kb;
+    uint32_t off;
+    float In, qxn, qyn, qzn;
+
+    __device__ __forceinline__ uint32_t offset(const PMK &P, int y, int x) const
+    {
+        return (__umul24((uint32_t)(y - P.r + ka * P.step), (uint32_t)P.W) + (uint32_t)(x - P.r + kb * P.step)) << 2;
+    }
+    __device__ __forceinline__ void load(const PMK &P)
+    {
+        const size_t HW = (size_t)P.H * P.W;
+        In = *(const float *)((const char *)P.ref + off);
+        qxn = *(const float *)((const char *)P.ray + off);
+        qyn = *(const float *)((const char *)(P.ray + HW) + off);
+        qzn = *(const float *)((const char *)(P.ray + 2 * HW) + off);
+    }
+    __device__ __forceinline__ void start(const PMK &P, int y, int x)
+    {
+        const int r = P.r, st = P.step, nw = 2 * r / st;
+        ka0 = y < r ? (r - y + st - 1) / st : 0;
+        const int ka1 = min(nw, (P.H - 1 - y + r) / st);
+        kb0 = x < r ? (r - x + st - 1) / st : 0;
+        kb1 = min(nw, (P.W - 1 - x + r) / st);
+        ns = (ka1 >= ka0 && kb1 >= kb0) ? (ka1 - ka0 + 1) * (kb1 - kb0 + 1) : 0;
+    }
+    // back to the first sample (each source group walks the window again)
+    __device__ __forceinline__ void rewind(const PMK &P, int y, int x)
+    {
+        ka = ka0;
+        kb = kb0;
+        off = ns > 0 ? offset(P, y, x) : 0u;
+        In = qxn = qyn = qzn = 0.f;
+        if (ns > 0)
+            load(P);
+    }
+    // sample j's values; then the load of sample j + 1 (sample j again after the last)
+    __device__ __forceinline__ void step(const PMK &P, int y, int x, int j, float &I, float &qx, float &qy, float &qz)
+    {
+        I = In;
+        qx = qxn;
+        qy = qyn;
+        qz = qzn;
+        if (j + 1 < ns) {
+            if (++kb > kb1) {
+                kb = kb0;
+                ++ka;
+            }
+            off = offset(P, y, x);
+        }
+        load(P);
+    }
+};
+
 constexpr int kGroupLP = 8;  // sources accumulated together by the precision variant
 
 // Precision variant of the photometric part of pm_cost: the plane (num = d n.ray_p) in FP64,
@@ -218,9 +278,10 @@ template <int FMT>
 __device__ __forceinline__ void pm_ncc_lp(const PMK &P, int y, int x, double num, double n0, double n1, double n2,
                                           double *ncc_l)
 {
-    const int H = P.H, W = P.W, r = P.r, st = P.step, S = P.S;
-    const size_t HW = (size_t)H * W;
+    const int S = P.S;
     const float numf = (float)num, nf0 = (float)n0, nf1 = (float)n1, nf2 = (float)n2;
+    PMWindow win;
+    win.start(P, y, x);
 #pragma unroll 1
     for (int g0 = 0; g0 < S; g0 += kGroupLP) {
         float sI[kGroupLP], sJ[kGroupLP], sII[kGroupLP], sJJ[kGroupLP], sIJ[kGroupLP];
@@ -231,52 +292,44 @@ __device__ __forceinline__ void pm_ncc_lp(const PMK &P, int y, int x, double num
             cnt[u] = 0;
         }
         int nsamp = 0;
-        for (int a = -r; a <= r; a += st) {
-            const int yy = y + a;
-            if (yy < 0 || yy >= H)
+        win.rewind(P, y, x);
+        for (int j = 0; j < win.ns; ++j) {
+            float I, qx, qy, qz;
+            win.step(P, y, x, j, I, qx, qy, qz);
+            if (!isfinite(I))
                 continue;
-            for (int b = -r; b <= r; b += st) {
-                const int xx = x + b;
-                if (xx < 0 || xx >= W)
-                    continue;
-                const size_t q = (size_t)yy * W + xx;
-                const float I = P.ref[q];
-                if (!isfinite(I))
-                    continue;
-                const float qx = P.ray[q], qy = P.ray[HW + q], qz = P.ray[2 * HW + q];
-                const float den = fmaf(nf0, qx, fmaf(nf1, qy, nf2 * qz));
-                if (den >= -1e-9f)
-                    continue;
-                const float t = numf * __builtin_amdgcn_rcpf(den);
-                nsamp += 1;
+            const float den = fmaf(nf0, qx, fmaf(nf1, qy, nf2 * qz));
+            if (den >= -1e-9f)
+                continue;
+            const float t = numf * __builtin_amdgcn_rcpf(den);
+            nsamp += 1;
 #pragma unroll
-                for (int u = 0; u < kGroupLP; ++u) {
-                    const int s = g0 + u;
-                    if (s < S) {
-                        const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
-                        const float dx = fmaf(t, qx, cs.bf[0]), dy = fmaf(t, qy, cs.bf[1]), dz = fmaf(t, qz, cs.bf[2]);
-                        const float zc = fmaf(cs.rf[6], dx, fmaf(cs.rf[7], dy, cs.rf[8] * dz));
-                        if (zc > 1e-6f) {
-                            const float xc = fmaf(cs.rf[0], dx, fmaf(cs.rf[1], dy, cs.rf[2] * dz));
-                            const float yc = fmaf(cs.rf[3], dx, fmaf(cs.rf[4], dy, cs.rf[5] * dz));
-                            const float iz = __builtin_amdgcn_rcpf(zc);
-                            const float pu = fmaf(cs.ff, xc * iz, cs.cxf), pv = fmaf(cs.ff, yc * iz, cs.cyf);
-                            if (pu >= 0.f && pv >= 0.f && pu <= cs.wlf && pv <= cs.hlf) {
-                                const int x0 = (int)pu, y0 = (int)pv;
-                                const float fx = pu - (float)x0, fy = pv - (float)y0;
-                                const uint16_t *pp = (const uint16_t *)cs.img + (size_t)y0 * cs.iw + x0;
-                                const float a00 = pm_ld16<FMT>(pp), a01 = pm_ld16<FMT>(pp + 1);
-                                const float a10 = pm_ld16<FMT>(pp + cs.iw), a11 = pm_ld16<FMT>(pp + cs.iw + 1);
-                                const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
-                                const float J = fmaf(fy, t1 - t0, t0);
-                                if (isfinite(J)) {
-                                    sI[u] += I;
-                                    sJ[u] += J;
-                                    sII[u] = fmaf(I, I, sII[u]);
-                                    sJJ[u] = fmaf(J, J, sJJ[u]);
-                                    sIJ[u] = fmaf(I, J, sIJ[u]);
-                                    cnt[u] += 1;
-                                }
+            for (int u = 0; u < kGroupLP; ++u) {
+                const int s = g0 + u;
+                if (s < S) {
+                    const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
+                    const float dx = fmaf(t, qx, cs.bf[0]), dy = fmaf(t, qy, cs.bf[1]), dz = fmaf(t, qz, cs.bf[2]);
+                    const float zc = fmaf(cs.rf[6], dx, fmaf(cs.rf[7], dy, cs.rf[8] * dz));
+                    if (zc > 1e-6f) {
+                        const float xc = fmaf(cs.rf[0], dx, fmaf(cs.rf[1], dy, cs.rf[2] * dz));
+                        const float yc = fmaf(cs.rf[3], dx, fmaf(cs.rf[4], dy, cs.rf[5] * dz));
+                        const float iz = __builtin_amdgcn_rcpf(zc);
+                        const float pu = fmaf(cs.ff, xc * iz, cs.cxf), pv = fmaf(cs.ff, yc * iz, cs.cyf);
+                        if (pu >= 0.f && pv >= 0.f && pu <= cs.wlf && pv <= cs.hlf) {
+                            const int x0 = (int)pu, y0 = (int)pv;
+                            const float fx = pu - (float)x0, fy = pv - (float)y0;
+                            const uint16_t *pp = (const uint16_t *)cs.img + (size_t)y0 * cs.iw + x0;
+                            const float a00 = pm_ld16<FMT>(pp), a01 = pm_ld16<FMT>(pp + 1);
+                            const float a10 = pm_ld16<FMT>(pp + cs.iw), a11 = pm_ld16<FMT>(pp + cs.iw + 1);
+                            const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
+                            const float J = fmaf(fy, t1 - t0, t0);
+                            if (isfinite(J)) {
+                                sI[u] += I;
+                                sJ[u] += J;
+                                sII[u] = fmaf(I, I, sII[u]);
+                                sJJ[u] = fmaf(J, J, sJJ[u]);
+                                sIJ[u] = fmaf(I, J, sIJ[u]);
+                                cnt[u] += 1;
                             }
                         }
                     }
@@ -314,26 +367,14 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
     if (ndr >= -1e-6 || d <= 0)
         return 10.0;
     const double num = d * ndr;
-    const int r = P.r, st = P.step, S = P.S;
+    const int S = P.S;
     double *ncc_l = pm_ncc_scratch();  // this thread's S NCC values (LDS)
     if (FMT != 0) {
         pm_ncc_lp<FMT>(P, y, x, num, n0, n1, n2, ncc_l);
     } else {
     int nsamp = 0;
-    // The window samples inside the image (the reference's loops skip the others,
-    // _dense.pyx:487-492): rows a = -r + ka st for ka in [ka0, ka1], columns likewise, walked row
-    // by row as one sequence of ns samples.
-    const int nw = 2 * r / st;
-    const int ka0 = y < r ? (r - y + st - 1) / st : 0, ka1 = min(nw, (H - 1 - y + r) / st);
-    const int kb0 = x < r ? (r - x + st - 1) / st : 0, kb1 = min(nw, (W - 1 - x + r) / st);
-    const int ns = (ka1 >= ka0 && kb1 >= kb0) ? (ka1 - ka0 + 1) * (kb1 - kb0 + 1) : 0;
-    // byte offset of window sample (ka, kb) in the reference image and in each ray plane (images
-    // of fewer than 2^30 pixels)
-    const char *refb = (const char *)P.ref, *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
-               *rayz = (const char *)(P.ray + 2 * HW);
-    auto sample_off = [&](int ka, int kb) -> uint32_t {
-        return (__umul24((uint32_t)(y - r + ka * st), (uint32_t)W) + (uint32_t)(x - r + kb * st)) << 2;
-    };
+    PMWindow win;
+    win.start(P, y, x);
     // Sources in groups of kGroup, the group loop kept rolled: per-sample geometry is recomputed
     // once per group, only kGroup sets of accumulators and one group's camera parameters are live.
     // Each source still sums its samples in window order, so every sum is the reference's.
@@ -347,30 +388,11 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
             cnt[u] = 0;
         }
         nsamp = 0;
-        // Each step loads the next sample's reference value and ray before it works on its own,
-        // so their latency overlaps the current sample's projections.
-        int ka = ka0, kb = kb0;
-        uint32_t off = ns > 0 ? sample_off(ka, kb) : 0u;
-        float In = 0.f, qxn = 0.f, qyn = 0.f, qzn = 0.f;
-        if (ns > 0) {
-            In = *(const float *)(refb + off);
-            qxn = *(const float *)(rayx + off);
-            qyn = *(const float *)(rayy + off);
-            qzn = *(const float *)(rayz + off);
-        }
-        for (int j = 0; j < ns; ++j) {
-            const double I = In, qx = qxn, qy = qyn, qz = qzn;
-            if (j + 1 < ns) {
-                if (++kb > kb1) {
-                    kb = kb0;
-                    ++ka;
-                }
-                off = sample_off(ka, kb);
-            }
-            In = *(const float *)(refb + off);
-            qxn = *(const float *)(rayx + off);
-            qyn = *(const float *)(rayy + off);
-            qzn = *(const float *)(rayz + off);
+        win.rewind(P, y, x);
+        for (int j = 0; j < win.ns; ++j) {
+            float If, qxf, qyf, qzf;
+            win.step(P, y, x, j, If, qxf, qyf, qzf);
+            const double I = If, qx = qxf, qy = qyf, qz = qzf;
             if (!isfinite(I))
                 continue;
             const double den = n0 * qx + n1 * qy + n2 * qz;
