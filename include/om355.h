@@ -322,12 +322,13 @@ int om_topk_mean(const float *st, int32_t n, int64_t M, int32_t k, float *out, v
 int om_argminmax0(const float *a, int32_t n, int64_t M, int32_t want_max, int64_t *idx, float *val, void *stream);
 
 /* Diagnostic, no counterpart in the reference: the two arithmetic shortcuts of the FP64 kernels,
- * exposed so that tests can pin them against IEEE arithmetic.  qa = a / c and qb = b / c as the
- * projection computes them (one shared reciprocal when the operands allow, plain division
- * otherwise; meant for c > 1e-6), fr = a - floor(a) as the bilinear tap's v_fract_f64.  Device
- * arrays of n doubles. */
-int om_fp64_steps(const double *a, const double *b, const double *c, int64_t n, double *qa, double *qb, double *fr,
-                  void *stream);
+ * exposed so that tests can pin them against IEEE arithmetic.  Quotients sharing one reciprocal
+ * of the divisor when the operands allow, plain division otherwise (meant for c > 1e-6): qa = a / c
+ * and qb = b / c as the projection computes its pair, qs = (a + b) / c as the middle one of the
+ * sweep NCC's three (b, a + b, a over c); fr = a - floor(a) as the bilinear tap's v_fract_f64.
+ * Device arrays of n doubles. */
+int om_fp64_steps(const double *a, const double *b, const double *c, int64_t n, double *qa, double *qb, double *qs,
+                  double *fr, void *stream);
 
 #ifdef __cplusplus
 }

@@ -76,27 +76,45 @@ __device__ __forceinline__ const T &uniform_ref(const T *p)
     return *(const T *)as_const(p);
 }
 
-// xc / zc and yc / zc for zc > 1e-6 (or NaN), each bit-identical to IEEE division.  When both
-// numerators lie in [2^-900, 2^100] in magnitude and zc <= 2^100, the two quotients share one
-// reciprocal of zc and run the steps of the compiler's own f64 division (v_rcp_f64, two Newton
-// steps, quotient, FMA residual, FMA correction): in that range its operand scaling
-// (v_div_scale_f64) and special-case fix-up (v_div_fixup_f64) return their inputs unchanged, so
-// the bits agree.  Zeros and tiny or huge operands take plain division; a NaN operand gives a NaN
-// quotient on either path.
+// x[i] / z for z > 1e-6 (or NaN), each bit-identical to IEEE division.  When every numerator lies
+// in [2^-900, 2^100] in magnitude and z <= 2^100, the quotients share one reciprocal of z and run
+// the steps of the compiler's own f64 division (v_rcp_f64, two Newton steps, quotient, FMA
+// residual, FMA correction): in that range its operand scaling (v_div_scale_f64) and special-case
+// fix-up (v_div_fixup_f64) return their inputs unchanged, so the bits agree.  Zeros and tiny or
+// huge operands take plain division; a NaN operand gives a NaN quotient on either path.
+template <int N>
+__device__ __forceinline__ void quotients_shared(const double (&x)[N], double z, double (&q)[N])
+{
+    double lo = fabs(x[0]), hi = fabs(x[0]);
+#pragma unroll
+    for (int i = 1; i < N; ++i) {
+        lo = fmin(lo, fabs(x[i]));
+        hi = fmax(hi, fabs(x[i]));
+    }
+    if (lo >= 0x1p-900 && hi <= 0x1p100 && z <= 0x1p100) {
+        double r = __builtin_amdgcn_rcp(z);
+        r = fma(r, fma(-z, r, 1.0), r);
+        r = fma(r, fma(-z, r, 1.0), r);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double q0 = x[i] * r;
+            q[i] = fma(fma(-z, q0, x[i]), r, q0);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            q[i] = x[i] / z;
+    }
+}
+
+// the projection's pair: xc / zc and yc / zc
 __device__ __forceinline__ void div2_shared(double xc, double yc, double zc, double &nx, double &ny)
 {
-    const double lo = fmin(fabs(xc), fabs(yc)), hi = fmax(fabs(xc), fabs(yc));
-    if (lo >= 0x1p-900 && hi <= 0x1p100 && zc <= 0x1p100) {
-        double r = __builtin_amdgcn_rcp(zc);
-        r = fma(r, fma(-zc, r, 1.0), r);
-        r = fma(r, fma(-zc, r, 1.0), r);
-        const double qx = xc * r, qy = yc * r;
-        nx = fma(fma(-zc, qx, xc), r, qx);
-        ny = fma(fma(-zc, qy, yc), r, qy);
-    } else {
-        nx = xc / zc;
-        ny = yc / zc;
-    }
+    const double x[2] = {xc, yc};
+    double q[2];
+    quotients_shared<2>(x, zc, q);
+    nx = q[0];
+    ny = q[1];
 }
 
 // Bilinear tap of a float32 image at 0 <= u <= w - 1.001, 0 <= v <= h - 1.001, in FP64 in the

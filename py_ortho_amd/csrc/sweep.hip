@@ -328,9 +328,10 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                         if (cov <= 0.9) {
                             out = -1.0f;
                         } else {
-                            const double mj = c1 * kk / cov;
-                            const double m2 = c2 * kk / cov;
-                            const double mij = c3 * kk / cov;
+                            const double num[3] = {c1 * kk, c2 * kk, c3 * kk};
+                            double mq[3];
+                            quotients_shared<3>(num, cov, mq);  // cov > 0.9: IEEE quotients
+                            const double mj = mq[0], m2 = mq[1], mij = mq[2];
                             double sj = m2 - mj * mj;
                             sj = sj > 1e-6 ? sqrt(sj) : 1e-3;
                             const double q = (mij - (double)mu_[j] * mj) / ((double)sd_[j] * sj);

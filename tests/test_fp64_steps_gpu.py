@@ -1,9 +1,10 @@
 """The FP64 kernels' two arithmetic shortcuts against IEEE double arithmetic, bit for bit.
 
-- The projection's two quotients xc / zc and yc / zc share one reciprocal of zc when the operands
-  lie in the range where the compiler's own division would neither scale its operands nor fix up
-  its result, and divide plainly otherwise (om_fp64_steps: qa, qb).  NumPy's float64 division is
-  IEEE correctly rounded, so it is the exact expectation.
+- Quotients over one divisor share its reciprocal when the operands lie in the range where the
+  compiler's own division would neither scale its operands nor fix up its result, and divide
+  plainly otherwise: the projection's pair xc / zc, yc / zc (om_fp64_steps: qa, qb) and the sweep
+  NCC's three means (qs, the middle of three).  NumPy's float64 division is IEEE correctly
+  rounded, so it is the exact expectation.
 - The bilinear tap's fraction u - (int)u, computed as v_fract_f64 (fr), for u >= 0 (the tap only
   runs inside [0, w - 1.001]).
 
@@ -26,11 +27,11 @@ N = 1 << 20
 
 def _run(a, b, c):
     A, B, C = (torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda() for v in (a, b, c))
-    qa, qb, fr = (torch.empty_like(A) for _ in range(3))
+    qa, qb, qs, fr = (torch.empty_like(A) for _ in range(4))
     check(_lib.lib().om_fp64_steps(A.data_ptr(), B.data_ptr(), C.data_ptr(), A.numel(), qa.data_ptr(),
-                                   qb.data_ptr(), fr.data_ptr(), _stream()), "om_fp64_steps")
+                                   qb.data_ptr(), qs.data_ptr(), fr.data_ptr(), _stream()), "om_fp64_steps")
     torch.cuda.synchronize()
-    return qa.cpu().numpy(), qb.cpu().numpy(), fr.cpu().numpy()
+    return qa.cpu().numpy(), qb.cpu().numpy(), qs.cpu().numpy(), fr.cpu().numpy()
 
 
 def _same(x, y):
@@ -66,12 +67,13 @@ def _operands(kind, rng):
 def test_projection_quotients_are_ieee_division(kind):
     rng = np.random.default_rng(["typical", "lower_edge", "upper_edge", "any"].index(kind) + 11)
     a, b, c = _operands(kind, rng)
-    qa, qb, _ = _run(a, b, c)
+    qa, qb, qs, _ = _run(a, b, c)
     with np.errstate(all="ignore"):
-        ea, eb = a / c, b / c
-    bad_a, bad_b = ~_same(qa, ea), ~_same(qb, eb)
+        ea, eb, es = a / c, b / c, (a + b) / c
+    bad_a, bad_b, bad_s = ~_same(qa, ea), ~_same(qb, eb), ~_same(qs, es)
     assert not bad_a.any(), (kind, a[bad_a][:3], c[bad_a][:3], qa[bad_a][:3], ea[bad_a][:3])
     assert not bad_b.any(), (kind, b[bad_b][:3], c[bad_b][:3], qb[bad_b][:3], eb[bad_b][:3])
+    assert not bad_s.any(), (kind, a[bad_s][:3], b[bad_s][:3], c[bad_s][:3], qs[bad_s][:3], es[bad_s][:3])
 
 
 def test_tap_fraction_is_u_minus_its_integer_part():
@@ -83,7 +85,7 @@ def test_tap_fraction_is_u_minus_its_integer_part():
         np.arange(N // 4, dtype=np.float64) / 4.0,         # exact quarters, whole numbers among them
     ])
     u[::1009] = np.nextafter(np.floor(u[::1009]) + 1.0, 0.0)  # just below the next integer
-    _, _, fr = _run(u, u, np.ones_like(u))
+    _, _, _, fr = _run(u, u, np.ones_like(u))
     exp = u - np.trunc(u)
     bad = ~_same(fr, exp)
     assert not bad.any(), (u[bad][:3], fr[bad][:3], exp[bad][:3])
