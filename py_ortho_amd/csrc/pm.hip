@@ -189,6 +189,7 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 #endif
 constexpr int kGroup = OM_PM_GROUP;  // sources accumulated together in the cost's sample loop
 constexpr int kPMThreads = 128;
+constexpr int kPassThreads = 1024;  // need / decide workgroups: one shared-counter update each
 constexpr int kSortBins = 65536;  // most tiles of source 0 the first-iteration item order uses
 
 // per-thread LDS slots for the S source scores of one cost evaluation
@@ -649,14 +650,14 @@ __device__ __forceinline__ void pm_scan_view(const PMK &P, const PassK &Q)
 
 // C: the 2 random refinements with the pixel's own slice of the stream (:646-664); score write
 template <int SM, int FMT>
-__device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64_t t)
+__device__ __forceinline__ int pm_random_px(const PMK &P, const PassK &Q, int64_t t)
 {
     int y, x;
     if (!colour_pixel(P, Q, t, y, x))
-        return;
+        return 0;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])
-        return;
+        return 0;
     double best = P.best[p], sc = P.sc[p];
     const int rk = P.rank[p];
     int changed = 0;
@@ -689,7 +690,7 @@ __device__ __forceinline__ void pm_random_px(const PMK &P, const PassK &Q, int64
     P.score[p] = (float)sc;
     if (P.chg && changed)
         P.chg[p] = 1;
-    count_changes(P, changed);
+    return changed;
 }
 
 // ---------------------------------------------------------------- memoised propagation
@@ -760,20 +761,35 @@ __device__ __forceinline__ void pm_need_px(const PMK &P, const PassK &Q, int64_t
         }
     }
     // slot-major order within the wave: the items of one slot over consecutive pixels sit together,
-    // so that a wave of the evaluation kernel mostly runs one neighbour direction in lockstep
-    const int lane = threadIdx.x & 63;
+    // so that a wave of the evaluation kernel mostly runs one neighbour direction in lockstep.  The
+    // waves of the workgroup take consecutive ranges of one reservation (one pair of atomics per
+    // workgroup: the shared counters are the contended part of this kernel).
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
-    int base = 0;
+    __shared__ int wbase[kPassThreads / 64];
     {
         int total = 0;
         for (int slot = 0; slot < 9; ++slot)
             total += __popcll(__ballot((mask >> slot) & 1u));
-        if (lane == 0 && total) {
-            base = atomicAdd(P.nitems, total);
-            atomicAdd(P.nevals, (unsigned long long)total);
+        if (lane == 0)
+            wbase[wave] = total;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                const int n = wbase[w];
+                wbase[w] = acc;
+                acc += n;
+            }
+            const int b0 = acc ? atomicAdd(P.nitems, acc) : 0;
+            if (acc)
+                atomicAdd(P.nevals, (unsigned long long)acc);
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+                wbase[w] += b0;
         }
-        base = __shfl(base, 0, 64);
+        __syncthreads();
     }
+    int base = wbase[wave];
     for (int slot = 0; slot < 9; ++slot) {
         const bool on = (mask >> slot) & 1u;
         const unsigned long long m = __ballot(on);
@@ -783,7 +799,7 @@ __device__ __forceinline__ void pm_need_px(const PMK &P, const PassK &Q, int64_t
     }
 }
 
-__global__ void __launch_bounds__(kPMThreads) pm_need_kernel(const PMK P, const PassK Q)
+__global__ void __launch_bounds__(kPassThreads) pm_need_kernel(const PMK P, const PassK Q)
 {
     pm_need_px(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
@@ -979,14 +995,14 @@ __device__ __forceinline__ void pm_eval_item(const PMK &P, const int *items, int
 }
 
 // (3) the reference's accept sequence over the memoised costs
-__device__ __forceinline__ void pm_decide_px(const PMK &P, const PassK &Q, int64_t t)
+__device__ __forceinline__ int pm_decide_px(const PMK &P, const PassK &Q, int64_t t)
 {
     int y, x;
     if (!colour_pixel(P, Q, t, y, x))
-        return;
+        return 0;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])
-        return;
+        return 0;
     const double *c = P.cc + p * 18;
     double best = c[16], sc = c[17];
     int changed = 0;
@@ -1010,12 +1026,30 @@ __device__ __forceinline__ void pm_decide_px(const PMK &P, const PassK &Q, int64
     P.best[p] = best;
     P.sc[p] = sc;
     P.chg[p] = changed ? 1 : 0;
-    count_changes(P, changed);
+    return changed;
 }
 
-__global__ void __launch_bounds__(kPMThreads) pm_decide_kernel(const PMK P, const PassK Q)
+// the workgroup's sum of v, in thread 0
+__device__ __forceinline__ int block_total(int v)
 {
-    pm_decide_px(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    __shared__ int part[kPassThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            s += part[w];
+    return s;
+}
+
+__global__ void __launch_bounds__(kPassThreads) pm_decide_kernel(const PMK P, const PassK Q)
+{
+    const int n = pm_decide_px(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    count_changes(P, block_total(n));  // one counter update per workgroup
 }
 
 // ---------------------------------------------------------------- single-view kernels
@@ -1032,7 +1066,8 @@ __global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P, const Pa
 template <int SM, int FMT>
 __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_kernel(const PMK P, const PassK Q)
 {
-    pm_random_px<SM, FMT>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    const int n = pm_random_px<SM, FMT>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    count_changes(P, block_total(n));
 }
 
 __global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed, const PMCamArgs A,
@@ -1074,7 +1109,9 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_batch(const P
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int v = view_of(blk0, nviews, b);
-    pm_random_px<SM, 0>(uniform_ref(views + v), Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
+    const PMK &P = uniform_ref(views + v);
+    const int n = pm_random_px<SM, 0>(P, Q, (int64_t)(b - as_const(blk0)[v]) * blockDim.x + threadIdx.x);
+    count_changes(P, block_total(n));
 }
 
 // item counters of every view; with clear_chg also the evaluation counts and the changed flags of
@@ -1204,7 +1241,7 @@ template <int SM, int FMT>
 static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uint32_t seed, const PMWs &w)
 {
     const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
-    const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
+    const unsigned gA = (unsigned)cdiv(ncol, kPMThreads), gN = (unsigned)cdiv(ncol, kPassThreads);
     P.cam = w.cam;
     P.st = w.st;
     const bool memo = w.cc != nullptr;
@@ -1235,14 +1272,14 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
                 ProfScope ps("pm_propagate", stream);
                 if (pass > 0)
                     hipLaunchKernelGGL(pm_memo_reset_kernel, dim3(1), dim3(64), 0, stream, P, HW, 0);
-                hipLaunchKernelGGL(pm_need_kernel, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                hipLaunchKernelGGL(pm_need_kernel, dim3(gN), dim3(kPassThreads), 0, stream, P, Q);
                 const PMK Pe = Q.first ? pm_sort_items(P, ncol, stream) : P;
                 {
                     ProfScope pe("pm_eval", stream);
                     hipLaunchKernelGGL((pm_eval_kernel<SM, FMT>), dim3((unsigned)eval_grid), dim3(kPMThreads), 0,
                                        stream, Pe);
                 }
-                hipLaunchKernelGGL(pm_decide_kernel, dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                hipLaunchKernelGGL(pm_decide_kernel, dim3(gN), dim3(kPassThreads), 0, stream, P, Q);
             } else {
                 ProfScope ps("pm_propagate", stream);
                 hipLaunchKernelGGL((pm_propagate_kernel<SM, FMT>), dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
@@ -1684,9 +1721,9 @@ static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, h
     for (int i = g.first; i < g.first + g.nv; ++i) {
         const PMK &P = host[i];
         const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
-        const unsigned gA = (unsigned)cdiv(ncol, kPMThreads);
+        const unsigned gN = (unsigned)cdiv(ncol, kPassThreads);
         const unsigned ge = (unsigned)(cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD);
-        hipLaunchKernelGGL(pm_need_kernel, dim3(gA), dim3(kPMThreads), 0, st, P, Q);
+        hipLaunchKernelGGL(pm_need_kernel, dim3(gN), dim3(kPassThreads), 0, st, P, Q);
         const PMK Pe = Q.first ? pm_sort_items(P, ncol, st) : P;
         switch (g.sm) {
         case 4: hipLaunchKernelGGL((pm_eval_kernel<4, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
@@ -1694,7 +1731,7 @@ static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, h
         case 12: hipLaunchKernelGGL((pm_eval_kernel<12, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
         default: hipLaunchKernelGGL((pm_eval_kernel<16, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
         }
-        hipLaunchKernelGGL(pm_decide_kernel, dim3(gA), dim3(kPMThreads), 0, st, P, Q);
+        hipLaunchKernelGGL(pm_decide_kernel, dim3(gN), dim3(kPassThreads), 0, st, P, Q);
     }
 }
 
