@@ -520,6 +520,34 @@ __device__ __forceinline__ bool colour_pixel(const PMK &P, const PassK &Q, int64
     return y < P.H && x < P.W;
 }
 
+// Pixels of a colour pass in tiles of kNeedTY rows x kNeedTX colour columns, one wave each, tiles in
+// row-major order: thread t of the need kernel (whose order the item list and so the evaluation
+// kernel's lanes follow) and of the random refinement.  A wave's windows then cover a compact
+// patch of the reference and of each source instead of a 128-pixel strip of one row (C2 229 ->
+// 214 ms with 4 x 16 tiles; 8 x 8 is as fast, 2 x 32 and 16 x 4 slower).
+#ifndef OM_NEED_TY
+#define OM_NEED_TY 4
+#endif
+constexpr int kNeedTY = OM_NEED_TY, kNeedTX = 64 / OM_NEED_TY;
+
+__host__ __device__ inline int64_t need_threads(int H, int W)
+{
+    const int64_t W2 = (W + 1) / 2;
+    return cdiv(H, kNeedTY) * cdiv(W2, kNeedTX) * 64;
+}
+
+__device__ __forceinline__ bool tile_pixel(const PMK &P, const PassK &Q, int64_t t, int &y, int &x)
+{
+    const int W2 = (P.W + 1) / 2, ntx = (W2 + kNeedTX - 1) / kNeedTX;
+    const int64_t tile = t / 64;
+    const int k = (int)(t - tile * 64);
+    const int ty = (int)(tile / ntx), tx = (int)(tile - (int64_t)ty * ntx);
+    y = ty * kNeedTY + k / kNeedTX;
+    const int j = tx * kNeedTX + k % kNeedTX;
+    x = 2 * j + ((Q.colour + y) & 1);
+    return y < P.H && j < W2 && x < P.W;
+}
+
 __device__ __forceinline__ void count_changes(const PMK &P, int n)
 {
     if (n)
@@ -653,7 +681,7 @@ template <int SM, int FMT>
 __device__ __forceinline__ int pm_random_px(const PMK &P, const PassK &Q, int64_t t)
 {
     int y, x;
-    if (!colour_pixel(P, Q, t, y, x))
+    if (!tile_pixel(P, Q, t, y, x))
         return 0;
     const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
     if (!P.active[p])
@@ -746,7 +774,7 @@ __device__ __forceinline__ void pm_need_px(const PMK &P, const PassK &Q, int64_t
     int y, x;
     unsigned mask = 0;
     size_t p = 0;
-    if (colour_pixel(P, Q, t, y, x)) {
+    if (tile_pixel(P, Q, t, y, x)) {
         p = (size_t)y * P.W + x;
         if (P.active[p]) {
             if (Q.first || P.chg[p])
@@ -1242,6 +1270,8 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
 {
     const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
     const unsigned gA = (unsigned)cdiv(ncol, kPMThreads), gN = (unsigned)cdiv(ncol, kPassThreads);
+    const unsigned gT = (unsigned)cdiv(need_threads(P.H, P.W), kPassThreads);
+    const unsigned gR = (unsigned)cdiv(need_threads(P.H, P.W), kPMThreads);
     P.cam = w.cam;
     P.st = w.st;
     const bool memo = w.cc != nullptr;
@@ -1272,7 +1302,7 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
                 ProfScope ps("pm_propagate", stream);
                 if (pass > 0)
                     hipLaunchKernelGGL(pm_memo_reset_kernel, dim3(1), dim3(64), 0, stream, P, HW, 0);
-                hipLaunchKernelGGL(pm_need_kernel, dim3(gN), dim3(kPassThreads), 0, stream, P, Q);
+                hipLaunchKernelGGL(pm_need_kernel, dim3(gT), dim3(kPassThreads), 0, stream, P, Q);
                 const PMK Pe = Q.first ? pm_sort_items(P, ncol, stream) : P;
                 {
                     ProfScope pe("pm_eval", stream);
@@ -1294,7 +1324,7 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
             }
             {
                 ProfScope ps("pm_random", stream);
-                hipLaunchKernelGGL((pm_random_kernel<SM, FMT>), dim3(gA), dim3(kPMThreads), 0, stream, P, Q);
+                hipLaunchKernelGGL((pm_random_kernel<SM, FMT>), dim3(gR), dim3(kPMThreads), 0, stream, P, Q);
             }
         }
     }
@@ -1679,8 +1709,7 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
             gblk0.push_back(0);
         }
         PMGroup &g = b->pm_groups[b->pm_ngroups - 1];
-        const int64_t ncol = (int64_t)Q.H * ((Q.W + 1) / 2);
-        g.blk += (int)cdiv(ncol, kPMThreads);
+        g.blk += (int)cdiv(need_threads(Q.H, Q.W), kPMThreads);  // random refinement walks tiles
         g.nv += 1;
         gblk0.push_back(g.blk);
     }
@@ -1722,8 +1751,9 @@ static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, h
         const PMK &P = host[i];
         const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
         const unsigned gN = (unsigned)cdiv(ncol, kPassThreads);
+        const unsigned gT = (unsigned)cdiv(need_threads(P.H, P.W), kPassThreads);
         const unsigned ge = (unsigned)(cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD);
-        hipLaunchKernelGGL(pm_need_kernel, dim3(gN), dim3(kPassThreads), 0, st, P, Q);
+        hipLaunchKernelGGL(pm_need_kernel, dim3(gT), dim3(kPassThreads), 0, st, P, Q);
         const PMK Pe = Q.first ? pm_sort_items(P, ncol, st) : P;
         switch (g.sm) {
         case 4: hipLaunchKernelGGL((pm_eval_kernel<4, 0>), dim3(ge), dim3(kPMThreads), 0, st, Pe); break;
