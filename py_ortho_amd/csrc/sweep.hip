@@ -224,13 +224,25 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             const double wl = (double)S.w - 1.001, hl = (double)S.h - 1.001;
             const float *img = S.img;
             const int sw = S.w;
-            // ---- W: warp the halo into LDS (_sweep.h:49-80)
+            // ---- W: warp the halo into LDS (_sweep.h:49-80).  Each step loads the ray of its next
+            // halo pixel (pixel 0's outside the image) before it works on its own.
+            const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
+                       *rayz = (const char *)(P.ray + 2 * HW);
+            int pn = tid < NHP ? sP[tid] : -1;
+            uint32_t on = (uint32_t)max(pn, 0) << 2;
+            float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
+                  rzn = *(const float *)(rayz + on);
             for (int hp = tid; hp < NHP; hp += kThreads) {
                 float J = NAN;
-                const int p = sP[hp];
+                const int p = pn;
+                const double rx = rxn, ry = ryn, rz = rzn;
+                pn = hp + kThreads < NHP ? sP[hp + kThreads] : -1;
+                on = (uint32_t)max(pn, 0) << 2;
+                rxn = *(const float *)(rayx + on);
+                ryn = *(const float *)(rayy + on);
+                rzn = *(const float *)(rayz + on);
                 if (p >= 0) {
                     const double d = CD ? dc : sD[hp];
-                    const double rx = P.ray[p], ry = P.ray[HW + p], rz = P.ray[2 * HW + p];
                     const double X = g[9] + d * ((g[0] * rx + g[1] * ry) + g[2] * rz);
                     const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
                     const double Z = g[11] + d * ((g[6] * rx + g[7] * ry) + g[8] * rz);
@@ -572,6 +584,8 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
         return fail(OM_EINVAL, "negative window radius");
     if (d->H < 0 || d->W < 0 || d->nh < 0)
         return fail(OM_ESHAPE, "negative size");
+    if ((int64_t)d->H * d->W >= kMaxTapPixels)
+        return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more are not supported");
     if (d->r > kMaxR)
         return fail(OM_EUNSUP, "fused sweep supports window radius <= %d (got %d)", kMaxR, d->r);
     if (d->top_k < 1)
