@@ -172,13 +172,25 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
         for (int s = 0; s < P.n; ++s) {
             const SrcLP &S = P.src[s];
             const float *g = S.g;
-            // ---- W: warp the halo (FP32 projection, FP32 bilinear of 16-bit taps)
+            // ---- W: warp the halo (FP32 projection, FP32 bilinear of 16-bit taps); each step loads
+            // the ray of its next halo pixel (pixel 0's outside the image) before it works on its own
+            const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
+                       *rayz = (const char *)(P.ray + 2 * HW);
+            int pn = tid < NH ? sP[tid] : -1;
+            uint32_t on = (uint32_t)max(pn, 0) << 2;
+            float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
+                  rzn = *(const float *)(rayz + on);
             for (int o = tid; o < NH; o += kThreads) {
-                const int p = sP[o];
+                const int p = pn;
+                const float rx = rxn, ry = ryn, rz = rzn;
+                pn = o + kThreads < NH ? sP[o + kThreads] : -1;
+                on = (uint32_t)max(pn, 0) << 2;
+                rxn = *(const float *)(rayx + on);
+                ryn = *(const float *)(rayy + on);
+                rzn = *(const float *)(rayz + on);
                 float J = NAN;
                 if (p >= 0) {
                     const float d = CD ? dc : sD[o];
-                    const float rx = P.ray[p], ry = P.ray[HW + p], rz = P.ray[2 * HW + p];
                     const float X = fmaf(d, fmaf(g[0], rx, fmaf(g[1], ry, g[2] * rz)), g[9]);
                     const float Y = fmaf(d, fmaf(g[3], rx, fmaf(g[4], ry, g[5] * rz)), g[10]);
                     const float Z = fmaf(d, fmaf(g[6], rx, fmaf(g[7], ry, g[8] * rz)), g[11]);
@@ -348,6 +360,8 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
         return fail(OM_EUNSUP, "reduced-precision sweep supports window radius 0..%d (got %d)", kMaxR, d->r);
     if (d->H < 0 || d->W < 0 || d->nh < 0 || d->i0 < 0)
         return fail(OM_ESHAPE, "negative size");
+    if ((int64_t)d->H * d->W >= kMaxTapPixels)
+        return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more are not supported");
     if (d->top_k < 1)
         return fail(OM_EINVAL, "top_k must be >= 1");
     const int k = d->top_k < d->n_src ? d->top_k : d->n_src;
