@@ -23,6 +23,7 @@
 // photometric score variable holding the previous evaluation's value, possibly another pixel's;
 // here it holds -1 at the start of each pixel.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include <algorithm>
@@ -1205,8 +1206,19 @@ struct PMWs {
     unsigned long long *nevals;
 };
 
-// the memo path lists items as pixel * 16 + slot in an int
-static bool pm_memo_ok(int H, int W) { return (int64_t)H * W < (1LL << 27); }
+// The memo path lists items as pixel * 16 + slot in an int, so it serves images below 2^27 pixels;
+// larger ones take the direct per-pixel kernels.  OM_PM_MEMO_MAX_PIXELS (environment, read at each
+// call) lowers that limit: the tests use it to run the direct path on small images.
+static bool pm_memo_ok(int H, int W)
+{
+    int64_t lim = 1LL << 27;
+    if (const char *e = getenv("OM_PM_MEMO_MAX_PIXELS")) {
+        const long long v = atoll(e);
+        if (v >= 0 && v < lim)
+            lim = v;
+    }
+    return (int64_t)H * W < lim;
+}
 
 // tiles of source 0 for the first-iteration item order: 16 pixels, larger for images beyond
 // kSortBins tiles

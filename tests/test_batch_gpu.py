@@ -118,6 +118,27 @@ def test_batch_pm_equals_per_view():
     B.close()
 
 
+def test_batch_pm_direct_path_equals_oracle(monkeypatch):
+    """Batched PatchMatch on the direct per-pixel kernels (forced by lowering the memo limit)."""
+    from py_ortho_amd import batch
+    monkeypatch.setenv("OM_PM_MEMO_MAX_PIXELS", "0")
+    r, step, k, gw, iters = 3, 2, 2, 0.5, 2
+    specs = [(61, 27, 33, 4, True), (62, 31, 26, 7, False)]
+    views, cs = _pm_views(specs)
+    B = batch.Batch()
+    B.set_pm(views, r, step, k, gw, iters)
+    B.run()
+    changed = B.pm_changed()
+    for V, c, ch in zip(views, cs, changed):
+        d0, n0, s0 = c["depth"].copy(), c["normal"].copy(), c["score"].copy()
+        ch0 = oracle.pm_refine(c["ref"], c["ray"], c["Cr"], c["srcs"], c["src_depths"], c["par"], d0, n0, s0,
+                               c["active"], r, step, k, gw, 0.02, iters, c["dmin"], c["dmax"], V.seed)
+        assert ch == ch0
+        assert _same(V.depth.cpu().numpy(), d0) and _same(V.normal.cpu().numpy(), n0)
+        assert _same(V.score.cpu().numpy(), s0)
+    B.close()
+
+
 def test_batch_graph_replays_match_eager():
     """Capture sweep + PatchMatch of a batch in a HIP graph; every replay (with eager device work
     and synchronisation between replays) reproduces the eager result exactly."""

@@ -72,6 +72,23 @@ def test_pm_refine_matches_oracle(S, wd, r, step, top_k, geo_w):
     assert _same(n, n0) and _same(s, s0) and ch == ch0
 
 
+@pytest.mark.parametrize("S,wd,r,step,top_k,geo_w", [(4, True, 5, 2, 2, 0.5), (12, False, 3, 1, 3, 0.0)])
+def test_pm_refine_direct_path_matches_oracle(monkeypatch, S, wd, r, step, top_k, geo_w):
+    """The direct per-pixel propagation kernel (images of 2^27 pixels and more, forced here by
+    lowering the memo limit) gives the oracle's bits too."""
+    monkeypatch.setenv("OM_PM_MEMO_MAX_PIXELS", "0")
+    c = make_case(60 + S, 23, 31, S, wd)
+    p = dict(r=r, step=step, top_k=top_k, geo_w=geo_w, geo_tol=0.02, dmin=c["dmin"], dmax=c["dmax"],
+             seed=777 + S)
+    d0, n0, s0 = c["depth"].copy(), c["normal"].copy(), c["score"].copy()
+    ch0 = oracle.pm_refine(c["ref"], c["ray"], c["Cr"], c["srcs"], c["src_depths"], c["par"], d0, n0, s0,
+                           c["active"], r, step, top_k, geo_w, 0.02, 2, c["dmin"], c["dmax"], p["seed"])
+    ch, d, n, s = _gpu_pm(c["ref"], c["ray"], c["Cr"], c["srcs"], c["src_depths"], c["par"], c["depth"],
+                          c["normal"], c["score"], c["active"], p, 2)
+    assert _same(d, d0), _report(d, d0)
+    assert _same(n, n0) and _same(s, s0) and ch == ch0
+
+
 def test_pm_refine_no_sources_returns_zero():
     from py_ortho_amd import dense
     z, srcs, deps, p = load_golden("s3")
