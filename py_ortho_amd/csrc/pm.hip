@@ -1039,7 +1039,7 @@ __device__ __forceinline__ int pm_decide_px(const PMK &P, const PassK &Q, int64_
         return 0;
     const double *c = P.cc + p * 18;
     double best = c[16], sc = c[17];
-    int changed = 0;
+    int changed = 0, last = -1;
     for (int k = 0; k < 8; ++k) {
         const int yn = y + kOffY[k], xn = x + kOffX[k];
         if (yn < 0 || xn < 0 || yn >= P.H || xn >= P.W)  // never listed, so never written
@@ -1047,15 +1047,20 @@ __device__ __forceinline__ int pm_decide_px(const PMK &P, const PassK &Q, int64_
         const double cst = c[2 * k];
         if (!(cst < best))  // NaN: no candidate from this neighbour
             continue;
-        double dp, b0, b1, b2;
-        pm_candidate(P, y, x, k, dp, b0, b1, b2);
         best = cst;
         sc = c[2 * k + 1];
+        last = k;
+        changed += 1;
+    }
+    // Each accepted candidate overwrites the plane and depends only on its neighbour, which this
+    // pass does not change, so only the last accepted one is formed and stored.
+    if (last >= 0) {
+        double dp, b0, b1, b2;
+        pm_candidate(P, y, x, last, dp, b0, b1, b2);
         P.depth[p] = (float)dp;
         P.normal[p] = (float)b0;
         P.normal[HW + p] = (float)b1;
         P.normal[2 * HW + p] = (float)b2;
-        changed += 1;
     }
     P.best[p] = best;
     P.sc[p] = sc;
