@@ -89,6 +89,29 @@ def test_pm_refine_direct_path_matches_oracle(monkeypatch, S, wd, r, step, top_k
     assert _same(n, n0) and _same(s, s0) and ch == ch0
 
 
+@pytest.mark.parametrize("direct", [False, True])
+@pytest.mark.parametrize("H,W", [(1, 1), (1, 6), (4, 1), (2, 3), (3, 7), (6, 11)])
+def test_pm_refine_tiny_views_match_oracle(monkeypatch, H, W, direct):
+    """Reference images of a few pixels (windows clipped on every side, colour tiles mostly empty),
+    on the memoised and the direct path."""
+    if direct:
+        monkeypatch.setenv("OM_PM_MEMO_MAX_PIXELS", "0")
+    c = make_case(90 + H * 10 + W, 9, 11, 4, True)
+    # a fresh array per call: a cut of whole rows would otherwise be a view of the case, and the
+    # oracle's in-place refinement would become the GPU's input
+    cut = lambda a: np.array(a[..., :H, :W], order="C", copy=True)  # noqa: E731
+    ref, ray, act = cut(c["ref"]), cut(c["ray"]), cut(c["active"])
+    act[:] = 1
+    p = dict(r=2, step=1, top_k=2, geo_w=0.5, geo_tol=0.02, dmin=c["dmin"], dmax=c["dmax"], seed=5)
+    d0, n0, s0 = cut(c["depth"]), cut(c["normal"]), cut(c["score"])
+    ch0 = oracle.pm_refine(ref, ray, c["Cr"], c["srcs"], c["src_depths"], c["par"], d0, n0, s0, act, p["r"],
+                           p["step"], p["top_k"], p["geo_w"], p["geo_tol"], 2, p["dmin"], p["dmax"], p["seed"])
+    ch, d, n, s = _gpu_pm(ref, ray, c["Cr"], c["srcs"], c["src_depths"], c["par"], cut(c["depth"]),
+                          cut(c["normal"]), cut(c["score"]), act, p, 2)
+    assert _same(d, d0), _report(d, d0)
+    assert _same(n, n0) and _same(s, s0) and ch == ch0
+
+
 def test_pm_refine_no_sources_returns_zero():
     from py_ortho_amd import dense
     z, srcs, deps, p = load_golden("s3")
