@@ -119,3 +119,40 @@ def test_pm_refine_no_sources_returns_zero():
     assert dense.pm_refine(_dev(z["ref"]), _dev(z["ray"]), z["Cr"], [], [], np.zeros((0, 18)), d, _dev(z["normal0"]),
                            _dev(z["score0"]), _dev(z["active"]), 4, 2, 2, 0.5, 0.02, 2, 1.0, 100.0, 1) == 0
     assert _same(d.cpu().numpy(), z["depth0"])
+
+
+def test_pm_refine_drone_scale_memo_equals_direct_and_repeats():
+    """At the drone-scale workload's full size (2000x1500 reference, 8 sources, 11x11 window with
+    step 2, geometric term, 2 of its 8 iterations): the memoised propagation and the direct one give
+    bit-identical depth, normal and score with the same change count, and a second run repeats the
+    first bit for bit.  The oracle would need minutes here; these properties do not depend on size."""
+    import os
+    from py_ortho_amd import dense, scene
+    sc = scene.config_c2(device="cuda")
+    H, W = sc.ref.img.shape
+    ray = scene.rays_world(sc.ref.cam, "cuda")
+    depth0, normal0 = scene.pm_init(3, H, W, ray, 80.0, 105.0)
+    depth0, normal0 = depth0.cuda(), normal0.cuda()
+    ref, srcs = sc.ref.img.contiguous(), [v.img.contiguous() for v in sc.srcs]
+    deps, par, Cr = [v.depth.contiguous() for v in sc.srcs], sc.par(), sc.ref.cam.C.copy()
+    active = torch.ones((H, W), dtype=torch.uint8, device="cuda")
+
+    def run():
+        d, n = depth0.clone(), normal0.clone()
+        s = torch.full((H, W), -1.0, dtype=torch.float32, device="cuda")
+        ch = dense.pm_refine(ref, ray, Cr, srcs, deps, par, d, n, s, active, 5, 2, 2, 0.5, 0.01, 2, 80.0, 105.0,
+                             2654435761)
+        return ch, d.cpu().numpy(), n.cpu().numpy(), s.cpu().numpy()
+
+    assert "OM_PM_MEMO_MAX_PIXELS" not in os.environ
+    a, b = run(), run()
+    os.environ["OM_PM_MEMO_MAX_PIXELS"] = "0"
+    try:
+        c = run()
+    finally:
+        del os.environ["OM_PM_MEMO_MAX_PIXELS"]
+    assert a[0] > H * W // 2  # most pixels change in the first iterations
+    for other in (b, c):
+        assert other[0] == a[0]
+        for k in (1, 2, 3):
+            assert _same(other[k], a[k]), (k, _report(other[k], a[k]))
