@@ -180,3 +180,37 @@ def test_batch_graph_replays_match_eager():
         assert all(_same(a, b) for a, b in zip(got, eager)), rep
         del junk
     B.close()
+
+
+@pytest.mark.parametrize("direct", [False, True])
+def test_batch_pm_with_tiny_views_equals_oracle(monkeypatch, direct):
+    """A batch that mixes reference images of a few pixels (1x1, 1x6, 4x1, 6x11, with windows
+    clipped on every side and mostly empty colour tiles) with an ordinary view: the per-view block,
+    row and pixel offsets of the batched kernels, on the memoised and the direct path."""
+    from py_ortho_amd import batch
+    if direct:
+        monkeypatch.setenv("OM_PM_MEMO_MAX_PIXELS", "0")
+    r, step, k, gw, iters = 2, 1, 2, 0.5, 2
+    views, cs = [], []
+    for j, (H, W) in enumerate([(1, 1), (1, 6), (30, 36), (4, 1), (6, 11)]):
+        c = make_case(70 + j, 9, 11, 4, True) if H < 9 else make_case(70 + j, H, W, 4, True)
+        cut = lambda a: np.array(a[..., :H, :W], order="C", copy=True)  # noqa: E731
+        c = dict(c, ref=cut(c["ref"]), ray=cut(c["ray"]), depth=cut(c["depth"]), normal=cut(c["normal"]),
+                 score=cut(c["score"]), active=np.ones((H, W), np.uint8))
+        views.append(batch.PMView(_dev(c["ref"]), _dev(c["ray"]), c["Cr"], [_dev(x) for x in c["srcs"]],
+                                  [None if x is None else _dev(x) for x in c["src_depths"]], c["par"],
+                                  _dev(c["depth"]), _dev(c["normal"]), _dev(c["score"]), _dev(c["active"]), 0.02,
+                                  c["dmin"], c["dmax"], 500 + j))
+        cs.append(c)
+    B = batch.Batch()
+    B.set_pm(views, r, step, k, gw, iters)
+    B.run()
+    changed = B.pm_changed()
+    for V, c, ch in zip(views, cs, changed):
+        d0, n0, s0 = c["depth"].copy(), c["normal"].copy(), c["score"].copy()
+        ch0 = oracle.pm_refine(c["ref"], c["ray"], c["Cr"], c["srcs"], c["src_depths"], c["par"], d0, n0, s0,
+                               c["active"], r, step, k, gw, 0.02, iters, c["dmin"], c["dmax"], V.seed)
+        assert ch == ch0, (c["ref"].shape, ch, ch0)
+        assert _same(V.depth.cpu().numpy(), d0) and _same(V.normal.cpu().numpy(), n0), c["ref"].shape
+        assert _same(V.score.cpu().numpy(), s0), c["ref"].shape
+    B.close()
