@@ -43,8 +43,12 @@ def _check_state(ref_st, ref_idx, st, idx, label):
                          f"idx mismatches {int((ref_idx != idx).sum())})")
 
 
+# (H, W, sources, r, top_k, hypotheses); the last five are one pixel, one row or one column wide
+# (r = 0 gives them scores other than -1; with r >= 1 every window there is under 90 % covered)
 @pytest.mark.parametrize("shape", cases.SWEEP_SHAPES + [(120, 150, 8, 5, 2, 6), (200, 333, 4, 2, 2, 12),
-                                                       (97, 140, 12, 7, 3, 5), (64, 64, 16, 1, 4, 4)])
+                                                       (97, 140, 12, 7, 3, 5), (64, 64, 16, 1, 4, 4),
+                                                       (1, 1, 2, 0, 1, 3), (1, 70, 3, 0, 2, 4), (70, 1, 3, 0, 2, 4),
+                                                       (3, 70, 2, 1, 2, 5), (1, 70, 3, 2, 2, 4)])
 def test_fused_sweep_matches_oracle(shape):
     H, W, n, r, k, nh = shape
     rng = np.random.default_rng(100 + H * 7 + W)
