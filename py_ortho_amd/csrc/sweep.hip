@@ -108,8 +108,8 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
 }
 
 // LDS rows of the halo are padded to an odd number of words and the horizontal sums to 65 words
-// per row, so that the horizontal pass (lanes spread over rows) and the vertical pass (lanes
-// spread over columns) hit distinct banks.
+// per row, so that the horizontal pass (a wave's lanes over 8 rows x 8 runs) and the vertical pass
+// (lanes spread over columns) hit distinct banks.
 constexpr int kTWP = kTW + 1;
 
 __host__ __device__ constexpr int halo_stride(int r) { return (kTW + 2 * r) | 1; }
@@ -268,12 +268,14 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 sJ[hp] = J;
             }
             __syncthreads();
-            // ---- H: horizontal window sums of one 8-column run per task (_sweep.h:85-114); the
-            // lanes of a wave take consecutive rows
+            // ---- H: horizontal window sums of one 8-column run per task (_sweep.h:85-114).  A
+            // wave's lanes take 8 rows x the 8 runs of each row: with the odd row strides their
+            // addresses fall in 64 distinct LDS banks (2- to 4-way conflicts when the lanes took
+            // consecutive rows of up to four runs)
             for (int t = tid; t < HR * kRunsPerRow; t += kThreads) {
-                const int hr = t % HR;
+                const int hr = t / kRunsPerRow;
                 const float *Jr = sJ + hr * HCP, *Ir = sI + hr * HCP;
-                const int c0 = (t / HR) * kRun;
+                const int c0 = (t % kRunsPerRow) * kRun;
                 int sv = 0;
                 double sj = 0.0, sjj = 0.0, sij = 0.0;
                 for (int t = 0; t <= 2 * r; ++t) {

@@ -213,9 +213,10 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                 sJ[o] = J;
             }
             __syncthreads();
-            // ---- H: FP32 running sums over 8-column runs; lanes spread over rows
+            // ---- H: FP32 running sums over 8-column runs; a wave's lanes take 8 rows x 8 runs
+            // (64 distinct LDS banks with the odd row strides)
             for (int t = tid; t < HR * (kTW / kRun); t += kThreads) {
-                const int hr = t % HR, c0 = (t / HR) * kRun;
+                const int hr = t / (kTW / kRun), c0 = (t % (kTW / kRun)) * kRun;
                 const float *Jr = sJ + hr * HCP, *Ir = sI + hr * HCP;
                 int sv = 0;
                 float sj = 0.f, sjj = 0.f, sij = 0.f;
