@@ -57,12 +57,12 @@ def test_fused_sweep_matches_oracle(shape):
     from py_ortho_amd import dense
     dsrc = [_dev(s) for s in srcs]
     maps = _dev(np.stack(hyps))
-    for chunks in (1, 0, 3):
+    for chunks, tile_h in ((1, 0), (0, 0), (3, 0), (2, 8)):  # 0: the default 16-row tiles
         st = dense.sweep(_dev(ref), _dev(mu), _dev(sd), dsrc, _dev(ray), geo, {"mode": "maps", "maps": maps}, r, k,
-                         chunks=chunks)
+                         chunks=chunks, tile_h=tile_h)
         torch.cuda.synchronize()
         a, ia = _state_np(st)
-        _check_state(st0, idx0, a, ia, (shape, chunks))
+        _check_state(st0, idx0, a, ia, (shape, chunks, tile_h))
 
 
 def test_sweep_hypothesis_per_call_matches_oracle():
