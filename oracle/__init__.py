@@ -60,6 +60,11 @@ def orc():
     return _orc
 
 
+def ref_built() -> bool:
+    """Whether oracle/_ref/libref.so exists (checked without loading it)."""
+    return os.path.exists(os.path.join(HERE, "_ref", "libref.so"))
+
+
 def ref_lib():
     """oracle/_ref/libref.so (the reference's _sweep.h), or None where it is not built."""
     global _ref

@@ -412,14 +412,18 @@ class DepthServer:
                 return
             px = sum(self._pixels(a) for a in self.active)
             got = []
+            # images of the requests taken in this pass are pinned only by prepare() below, so each
+            # candidate is checked against the images of every request taken before it as well
+            pending = set()
             while self.adm.queue and len(got) < room:
                 a = self.adm.queue[0]
                 p = self._pixels(a)
                 if (self.active or got) and px + p > self.max_active_pixels:
                     break
-                if not self.adm.take(lambda x: x.need, lambda x: self.store.pinned_bytes_with(x.keys), 1):
+                if not self.adm.take(lambda x: x.need, lambda x: self.store.pinned_bytes_with(pending | set(x.keys)), 1):
                     break
                 px += p
+                pending |= set(a.keys)
                 got.append(a)
         for a in got:
             try:

@@ -230,3 +230,22 @@ def test_pixel_limit_bounds_the_step_but_never_starves():
     assert max(len(s) for s in ex.steps) <= 2
     assert srv.stats["finished"] == 7
 
+
+
+def test_images_of_requests_admitted_together_count_against_the_budget():
+    """Image bytes dominate: two requests whose distinct sources do not fit together must not be
+    admitted in the same pass (the FakeExecutor asserts the budget when it pins)."""
+    sizes = {f"v{i}": (100, 100) for i in range(40)}  # image 40 kB
+    ws = FakeExecutor.working_bytes(100, 100, 8)       # 480 kB working set
+    srcs_a = [f"v{i}" for i in range(1, 17)]
+    srcs_b = [f"v{i}" for i in range(21, 37)]
+    one = ws + 17 * 40000                              # one request with its 17 images
+    budget = one + ws + 10 * 40000                     # the second's working set fits, its images do not
+    srv, store, ex = _server(sizes, budget, step_planes=8)
+    fa = srv.submit(DepthRequest("v0", srcs_a, 10.0, 20.0, 8, tag="a"))
+    fb = srv.submit(DepthRequest("v20", srcs_b, 10.0, 20.0, 8, tag="b"))
+    srv.start()
+    assert fa.result(timeout=10).tag == "a" and fb.result(timeout=10).tag == "b"
+    srv.stop()
+    assert max(len(s) for s in ex.steps) == 1            # never in flight together
+    assert srv.stats["finished"] == 2 and srv.stats["failed"] == 0
