@@ -95,6 +95,16 @@ int om_device_info(char *name, int32_t name_len, int32_t *cus, int32_t *clock_kh
 /* fused plane sweep: hypotheses [i0, i0+nh) of one view in one launch (+ a merge when chunked) */
 size_t om_sweep_workspace_bytes(const om_sweep_desc *d);
 int om_sweep(const om_sweep_desc *d, void *stream);
+/* om_sweep that allocates nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same descriptor and results as om_sweep; d->workspace is ignored and the hypothesis
+ * chunks' partial winner states go to a library-owned static device array of
+ * om_sweep_noalloc_max_bytes() bytes (64 MiB) per GPU.  Returns OM_ESHAPE when the chosen chunks
+ * need more than that (om_sweep_workspace_bytes(d) > the maximum; pass chunks = 1 to need none).
+ * Every call on one GPU shares that GPU's array: calls that overlap in time on one GPU (other
+ * streams or threads) race and must be serialised by the caller. */
+size_t om_sweep_noalloc_max_bytes(void);
+int om_sweep_noalloc(const om_sweep_desc *d, void *stream);
 
 /* _dense.sweep_hypothesis, one hypothesis i with an explicit (H, W) inverse-depth map `inv`:
  * imgs/img_hw: n device image pointers and their (h, w); geo: host (n, 18) float64. */

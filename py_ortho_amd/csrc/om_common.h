@@ -141,7 +141,7 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(const char *n, hipStream_t s)
     {
-        if (!prof_on())
+        if (!n || !prof_on())
             return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
             return;

@@ -32,6 +32,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "om_common.h"
 
@@ -672,43 +673,18 @@ static int choose_chunks(const om_sweep_desc *d, const SweepK &P)
     return c;
 }
 
-}  // namespace om
-
-using namespace om;
-
-extern "C" size_t om_sweep_workspace_bytes(const om_sweep_desc *d)
+// The sweep of one view over hypotheses [i0, i0 + nh) in `chunks` hypothesis chunks, whose
+// partial winner states go to `part` (chunks * 6 * H * W floats) when chunks > 1.
+static int sweep_launch(SweepK &Pl, int chunks, float *part, hipStream_t st, bool prof = true)
 {
-    SweepK P;
-    if (build_params(d, P) != OM_OK)
-        return 0;
-    const int c = choose_chunks(d, P);
-    if (c <= 1)
-        return 0;
-    return (size_t)c * kPartPlanes * (size_t)d->H * d->W * sizeof(float);
-}
-
-extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
-{
-    SweepK Pl;
-    int rc = build_params(d, Pl);
-    if (rc)
-        return rc;
-    if (Pl.H == 0 || Pl.W == 0 || Pl.nh == 0)
-        return OM_OK;
-    int chunks = choose_chunks(d, Pl);
-    if (chunks > 1) {
-        const size_t need = (size_t)chunks * kPartPlanes * (size_t)Pl.H * Pl.W * sizeof(float);
-        if (!d->workspace || d->workspace_bytes < need)
-            chunks = 1;  // no room for partials: one chunk
-    }
+    int rc;
     Pl.hc = (int)cdiv(Pl.nh, chunks);
     Pl.chunks = (int)cdiv(Pl.nh, Pl.hc);
-    Pl.part = (float *)d->workspace;
+    Pl.part = part;
     const bool cd = Pl.mode == OM_HYP_CONST;
     const size_t lds = sweep_lds(Pl.TH, Pl.r, cd);
     const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
-    hipStream_t st = as_stream(stream);
-    ProfScope ps("sweep", st);
+    ProfScope ps(prof ? "sweep" : nullptr, st);
     using KFn = void (*)(const SweepK);
     static const KFn table[4][2][2] = {
         {{sweep_kernel<1, 2, false>, sweep_kernel<1, 2, true>}, {sweep_kernel<1, 4, false>, sweep_kernel<1, 4, true>}},
@@ -720,13 +696,97 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
         return rc;
     if (Pl.chunks > 1) {
         const int64_t HW = (int64_t)Pl.H * Pl.W;
-        ProfScope pm("sweep_merge", st);
+        ProfScope pm(prof ? "sweep_merge" : nullptr, st);
         hipLaunchKernelGGL(sweep_merge_kernel, dim3((unsigned)cdiv(HW, 256)), dim3(256), 0, st, Pl.part, Pl.chunks,
                            HW, Pl.best, Pl.prev, Pl.spb, Pl.snb, Pl.idx, Pl.i0, Pl.hc);
         if ((rc = launch_check("sweep_merge_kernel")))
             return rc;
     }
     return OM_OK;
+}
+
+static size_t part_bytes(int chunks, const SweepK &P)
+{
+    return chunks > 1 ? (size_t)chunks * kPartPlanes * (size_t)P.H * P.W * sizeof(float) : 0;
+}
+
+// Chunk partials of om_sweep_noalloc: one array per GPU (each device that loads the library holds
+// its own copy until the process ends).  Host code must not pass the array's name to a kernel (it
+// is a host-side stand-in); it uses the device address from hipGetSymbolAddress, looked up once per
+// device on the first call there.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+static int scratch_ptr(void **out)
+{
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc)
+        return rc;
+    if (dev < 0 || dev >= 64)
+        return fail(OM_EUNSUP, "om_sweep_noalloc: device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if ((rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")))
+            return rc;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return OM_OK;
+}
+
+}  // namespace om
+
+using namespace om;
+
+extern "C" size_t om_sweep_workspace_bytes(const om_sweep_desc *d)
+{
+    SweepK P;
+    if (build_params(d, P) != OM_OK)
+        return 0;
+    return part_bytes(choose_chunks(d, P), P);
+}
+
+extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
+{
+    SweepK Pl;
+    int rc = build_params(d, Pl);
+    if (rc)
+        return rc;
+    if (Pl.H == 0 || Pl.W == 0 || Pl.nh == 0)
+        return OM_OK;
+    int chunks = choose_chunks(d, Pl);
+    if (chunks > 1 && (!d->workspace || d->workspace_bytes < part_bytes(chunks, Pl)))
+        chunks = 1;  // no room for partials: one chunk
+    return sweep_launch(Pl, chunks, (float *)d->workspace, as_stream(stream));
+}
+
+extern "C" size_t om_sweep_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int om_sweep_noalloc(const om_sweep_desc *d, void *stream)
+{
+    SweepK Pl;
+    int rc = build_params(d, Pl);
+    if (rc)
+        return rc;
+    const int chunks = choose_chunks(d, Pl);
+    const size_t need = part_bytes(chunks, Pl);
+    if (need > kNoAllocScratchBytes)
+        return fail(OM_ESHAPE,
+                    "om_sweep_noalloc: %d hypothesis chunks of a %dx%d view need %zu bytes of partials, more "
+                    "than its %zu-byte scratch (pass fewer chunks, or om_sweep with a workspace)",
+                    chunks, Pl.H, Pl.W, need, kNoAllocScratchBytes);
+    if (Pl.H == 0 || Pl.W == 0 || Pl.nh == 0)
+        return OM_OK;
+    void *part = nullptr;
+    if (chunks > 1 && (rc = scratch_ptr(&part)))
+        return rc;
+    // no profiling scopes here: they create HIP events
+    return sweep_launch(Pl, chunks, (float *)part, as_stream(stream), false);
 }
 
 extern "C" int om_sweep_hypothesis(const float *ref, const float *mu, const float *sd, int32_t H, int32_t W,
