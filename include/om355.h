@@ -201,8 +201,10 @@ int om_batch_pm_evaluated(om_batch *b, int64_t *evaluated);
 /* One step of a continuous-batching server: for every view, hypotheses [i0, i0 + nh) continuing its
  * winner state (views of any size, source count and hypothesis generator; same r and min(top_k, n)),
  * in one persistent launch per tile height; views with depth_out[v] != NULL finish (their i0 + nh is
- * the whole sweep) and get the parabola depth.  Equal, view by view, to om_sweep over the same range
- * followed by om_sweep_finalize.  The call allocates nothing: table is device memory and staging
+ * the whole sweep) and get the parabola depth.  The parabola reads hypotheses 0 and 1, so for a
+ * finishing view with i0 > 0 and OM_HYP_CONST or OM_HYP_MAPS, hyp_a must point i0 entries (maps)
+ * into a table that starts at hypothesis 0: hyp_a[-i0 ...] is read.  Equal, view by view, to om_sweep
+ * over the same range followed by om_sweep_finalize.  The call allocates nothing: table is device memory and staging
  * pinned host memory (hipHostMalloc), both of om_sweep_multi_bytes(nviews); the table is uploaded from
  * staging on `stream`, so staging may be reused once the stream has passed this call.  Replaces the
  * per-view thread pool of densify.py:596 for a stream of requests. */

@@ -135,6 +135,13 @@ __device__ __forceinline__ double bil_f64(const float *im, int w, double u, doub
 // most pixels an image read by bil_f64 may hold (32-bit byte offsets)
 constexpr int64_t kMaxTapPixels = int64_t(1) << 30;
 
+// Sizes an image tapped through 32-bit offsets may have: fewer than 2^30 pixels, and each side
+// below 2^24 (row offsets are formed with 24-bit multiplies, __umul24).
+inline bool tap_size_ok(int64_t h, int64_t w)
+{
+    return h < (int64_t(1) << 24) && w < (int64_t(1) << 24) && h * w < kMaxTapPixels;
+}
+
 struct ProfScope {
     const char *name = nullptr;
     hipStream_t st = nullptr;

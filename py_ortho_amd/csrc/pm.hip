@@ -1387,8 +1387,8 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
         return fail(OM_EINVAL, "null host array");
     if ((int64_t)H * W == 0 || iters <= 0)
         return OM_OK;
-    if ((int64_t)H * W >= kMaxTapPixels)
-        return fail(OM_ESHAPE, "pm_refine: reference images of 2^30 pixels or more are not supported");
+    if (!tap_size_ok(H, W))
+        return fail(OM_ESHAPE, "pm_refine: reference images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported");
     const bool memo = pm_memo_ok(H, W);
     const size_t need = pm_ws_layout(H, W, nullptr, nullptr, memo);
     if (!workspace || workspace_bytes < need)
@@ -1430,8 +1430,8 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
     for (int s = 0; s < S; ++s) {
         if (ih[s] < 1 || iw[s] < 1)
             return fail(OM_ESHAPE, "source %d has an empty image", s);
-        if ((int64_t)ih[s] * iw[s] >= kMaxTapPixels)
-            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more are not supported", s);
+        if (!tap_size_ok(ih[s], iw[s]))
+            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported", s);
         PMCam &c = A.cam[s];
         c.img = (const float *)imgs[s];
         c.dep = deps[s];
@@ -1634,8 +1634,8 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
             return fail(OM_ENSRC, "view %d: PatchMatch needs 1..16 sources", v);
         if (V.H < 1 || V.W < 1)
             return fail(OM_ESHAPE, "view %d: empty image", v);
-        if ((int64_t)V.H * V.W >= kMaxTapPixels)
-            return fail(OM_ESHAPE, "view %d: reference images of 2^30 pixels or more are not supported", v);
+        if (!tap_size_ok(V.H, V.W))
+            return fail(OM_ESHAPE, "view %d: reference images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported", v);
         char *ws = (char *)b->dev_alloc(pm_ws_layout(V.H, V.W, nullptr, nullptr, memo));
         if (!ws)
             return fail(OM_ENOMEM, "batch: PatchMatch workspace");
@@ -1649,8 +1649,8 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         for (int s = 0; s < V.S; ++s) {
             if (V.ih[s] < 1 || V.iw[s] < 1)
                 return fail(OM_ESHAPE, "view %d source %d has an empty image", v, s);
-            if ((int64_t)V.ih[s] * V.iw[s] >= kMaxTapPixels)
-                return fail(OM_ESHAPE, "view %d source %d: images of 2^30 pixels or more are not supported", v, s);
+            if (!tap_size_ok(V.ih[s], V.iw[s]))
+                return fail(OM_ESHAPE, "view %d source %d: images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported", v, s);
             cams[s].img = V.imgs[s];
             cams[s].dep = V.deps[s];
             cams[s].ih = V.ih[s];

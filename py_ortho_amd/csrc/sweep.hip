@@ -601,8 +601,8 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
         return fail(OM_EINVAL, "negative window radius");
     if (d->H < 0 || d->W < 0 || d->nh < 0)
         return fail(OM_ESHAPE, "negative size");
-    if ((int64_t)d->H * d->W >= kMaxTapPixels)
-        return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more are not supported");
+    if (!tap_size_ok(d->H, d->W))
+        return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported");
     if (d->r > kMaxR)
         return fail(OM_EUNSUP, "fused sweep supports window radius <= %d (got %d)", kMaxR, d->r);
     if (d->top_k < 1)
@@ -639,8 +639,8 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
     P.pinhole = 1;
     for (int s = 0; s < d->n_src; ++s) {
         const om_src &o = d->src[s];
-        if ((int64_t)o.h * o.w >= kMaxTapPixels)
-            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more are not supported", s);
+        if (!tap_size_ok(o.h, o.w))
+            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported", s);
         P.src[s].img = o.img;
         P.src[s].h = o.h;
         P.src[s].w = o.w;

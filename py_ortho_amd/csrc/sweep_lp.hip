@@ -362,8 +362,8 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
         return fail(OM_EUNSUP, "reduced-precision sweep supports window radius 0..%d (got %d)", kMaxR, d->r);
     if (d->H < 0 || d->W < 0 || d->nh < 0 || d->i0 < 0)
         return fail(OM_ESHAPE, "negative size");
-    if ((int64_t)d->H * d->W >= kMaxTapPixels)
-        return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more are not supported");
+    if (!tap_size_ok(d->H, d->W))
+        return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported");
     if (d->top_k < 1)
         return fail(OM_EINVAL, "top_k must be >= 1");
     const int k = d->top_k < d->n_src ? d->top_k : d->n_src;
@@ -397,6 +397,9 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
         const om_src &S = d->src[s];
         if (S.h < 1 || S.w < 1 || !S.img)
             return fail(OM_ESHAPE, "source %d is empty", s);
+        if (!tap_size_ok(S.h, S.w))
+            return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more, or with a side of 2^24 or more, are "
+                                   "not supported", s);
         if (S.geo[13] != 0.0 || S.geo[14] != 0.0 || S.geo[15] != 0.0)
             return fail(OM_EUNSUP, "reduced-precision sweep supports distortion-free sources only");
         SrcLP &Q = P.src[s];

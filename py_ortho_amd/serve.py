@@ -295,7 +295,9 @@ class GpuExecutor:
         keep = []
         for j, (a, i0, nh, fin) in enumerate(items):
             s = a.state
-            hyp_a = s["planes"][i0:]              # const planes: the table starts at hypothesis i0
+            # const planes: hyp_a points at hypothesis i0 of the whole table, whose entries 0 and 1
+            # the parabola of a finishing step reads (hyp_a[-i0], hyp_a[1 - i0])
+            hyp_a = s["planes"][i0:]
             d, k = dense._sweep_desc(s["ref"], s["mu"], s["sd"], s["srcs"], s["ray"], s["geo"], self.window,
                                      self.top_k, s["st"], _lib.OM_HYP_CONST, hyp_a, None, 0, i0, nh, 0, self.tile_h)
             descs[j] = d

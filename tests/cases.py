@@ -69,3 +69,29 @@ def sweep_case(rng, H, W, n, r, nhyp, nan_frac=0.02, distort=True):
 # (H, W, n, r, top_k, n_hyp): the shapes of the reference's parity test plus larger tiles
 SWEEP_SHAPES = [(37, 53, 4, 5, 2, 9), (64, 48, 1, 3, 2, 5), (11, 9, 3, 5, 2, 4), (5, 4, 2, 5, 1, 3),
                 (30, 31, 5, 2, 3, 6), (2, 40, 2, 1, 2, 3), (40, 3, 4, 4, 4, 3)]
+
+
+def np_init_normals(D, ray):
+    """densify._init_normals (densify.py:481-496) in NumPy's float32 operations, each written out
+    in the order NumPy evaluates them (np.cross products then differences, the norm as
+    (x0^2 + x1^2) + x2^2); equal bit for bit to the reference's expression on the golden survey
+    (tests/test_oracle.py)."""
+    D = np.asarray(D, np.float32)
+    ray = np.asarray(ray, np.float32)
+    P = ray * np.where(np.isfinite(D), D, np.nan)[None]
+    tu = np.full_like(P, np.nan)
+    tv = np.full_like(P, np.nan)
+    tu[:, :, 2:-2] = P[:, :, 4:] - P[:, :, :-4]
+    tv[:, 2:-2, :] = P[:, 4:, :] - P[:, :-4, :]
+    n0 = tu[1] * tv[2] - tu[2] * tv[1]
+    n1 = tu[2] * tv[0] - tu[0] * tv[2]
+    n2 = tu[0] * tv[1] - tu[1] * tv[0]
+    with np.errstate(invalid="ignore"):
+        nn = np.sqrt((n0 * n0 + n1 * n1) + n2 * n2)
+        bad = ~np.isfinite(nn) | (nn < 1e-12)
+        d = np.where(bad, np.float32(1), nn)
+        n = np.stack([n0 / d, n1 / d, n2 / d])
+    n[:, bad] = np.array([0.0, 0.0, 1.0], np.float32)[:, None]
+    flip = ((n[0] * ray[0] + n[1] * ray[1]) + n[2] * ray[2]) > 0
+    n[:, flip] *= -1
+    return np.ascontiguousarray(n, np.float32)

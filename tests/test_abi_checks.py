@@ -5,7 +5,7 @@ are never dereferenced.
 The reference's own contract (_dense.pyx): pm_refine returns 0 without work when the source count
 is outside 1..16 (:587-588); sweep_hypothesis raises ValueError for a source count outside 1..16
 or a negative radius (:759-771) and returns early for an empty image (:772-773).  The library adds
-the limits of its own kernels: images of fewer than 2^30 pixels, window radius <= 7 and
+the limits of its own kernels: images of fewer than 2^30 pixels with sides below 2^24, window radius <= 7 and
 min(top_k, n) <= 4 in the fused sweep (OM_EUNSUP), tile heights of 8 or 16 rows.
 """
 import ctypes
@@ -58,6 +58,8 @@ def test_pm_refine_empty_image_or_no_iterations_is_a_no_op():
     (dict(nbytes=255), _lib.OM_EINVAL, "workspace"),
     (dict(ih=0), _lib.OM_ESHAPE, "empty"),
     (dict(ih=1 << 15, iw=1 << 15), _lib.OM_ESHAPE, "2^30"),
+    (dict(H=1, W=1 << 24), _lib.OM_ESHAPE, "2^24"),          # a side past the 24-bit row multiply
+    (dict(ih=1, iw=1 << 24), _lib.OM_ESHAPE, "2^24"),
 ])
 def test_pm_refine_rejects_bad_arguments(kw, code, text):
     rc, changed = _pm(**kw)
@@ -92,6 +94,8 @@ def test_sweep_empty_image_or_no_hypotheses_is_a_no_op():
     (dict(H=-1), _lib.OM_ESHAPE),
     (dict(H=1 << 15, W=1 << 15), _lib.OM_ESHAPE),
     (dict(src_hw=(1 << 15, 1 << 15)), _lib.OM_ESHAPE),
+    (dict(H=1 << 24, W=1), _lib.OM_ESHAPE),
+    (dict(src_hw=(1 << 24, 1)), _lib.OM_ESHAPE),
     (dict(top_k=0), _lib.OM_EINVAL),
     (dict(mode=99), _lib.OM_EINVAL),
     (dict(null_src=True), _lib.OM_EINVAL),

@@ -95,3 +95,20 @@ def test_sample_view_matches_oracle_and_known_answer():
     out, valid = ops.sample_view(_dev(img), (Rn, np.array([0.0, 0.0, 10.0]), f, 0.0, 0.0, 0.0, 49.5, 49.5),
                                  X - 0.005, Y + 0.005, 0.01, _dev(np.zeros((1, 1), np.float32)))
     assert int(valid[0, 0]) == 1 and abs(float(out[0, 0, 0]) - 1.0) < 1e-4
+
+
+def _sample_view_golden():
+    import test_oracle
+    return list(test_oracle.sample_view_cases())
+
+
+@pytest.mark.parametrize("case", _sample_view_golden(), ids=lambda c: c[0])
+def test_sample_view_matches_reference_golden(case):
+    """om_sample_view == the compiled reference's CPUBackend.sample_view (_mvs.pyx:13-54,
+    backend.py:67-77), bit for bit: distorted, tilted and oblique cameras, (D, H, W) stacks, cells
+    behind the camera and on the last pixel row and column."""
+    from py_ortho_amd import ops
+    name, img, cam, X0, Y0, gsd, Z, want, wvalid = case
+    out, valid = ops.sample_view(_dev(img), cam, X0, Y0, gsd, _dev(Z))
+    assert np.array_equal(valid.cpu().numpy(), wvalid)
+    assert _same(out.cpu().numpy(), want)

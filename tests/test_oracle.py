@@ -280,3 +280,39 @@ def test_pm_refine_planar_fixed_point():
     rel = np.abs(depth[inner] - d0[inner]) / d0[inner]
     assert np.nanmedian(rel) < 1e-3 and np.nanmean(rel < 5e-3) > 0.95
     assert np.nanmedian(score[inner]) > 0.8
+
+
+# ------------------------------------------------------------------ height-sweep sampler
+SV_GOLDEN = os.path.join(GOLDEN, "sample_view.npz")
+
+
+def sample_view_cases():
+    """(name, img, cam, X0, Y0, gsd, Z, out, valid) from the compiled reference's CPUBackend.sample_view
+    (tests/golden/make_sample_view_golden.py)."""
+    z = np.load(SV_GOLDEN)
+    for name in z["names"]:
+        p = z[f"{name}_par"]
+        cam = (z[f"{name}_R"], z[f"{name}_C"], *p[:6])
+        yield (str(name), z[f"{name}_img"], cam, p[6], p[7], p[8], z[f"{name}_Z"], z[f"{name}_out"],
+               z[f"{name}_valid"])
+
+
+@pytest.mark.parametrize("case", list(sample_view_cases()), ids=lambda c: c[0])
+def test_sample_view_matches_reference_golden(case):
+    """orc_sample_view == _mvs.sample_view (_mvs.pyx:13-54) bit for bit: distortion k1..k3 != 0,
+    tilted and oblique cameras, fractional positions, (D, H, W) stacks, points behind the camera,
+    and cells exactly on the last pixel row and column (the x1 = x0, y1 = y0 clamps)."""
+    name, img, cam, X0, Y0, gsd, Z, want, wvalid = case
+    out, valid = oracle.sample_view(img, cam, X0, Y0, gsd, Z)
+    assert np.array_equal(valid, wvalid)
+    assert _same(out, want)
+    assert valid.any() and not valid.all()
+
+
+def test_np_init_normals_matches_reference_golden():
+    """cases.np_init_normals (the checker of the device normals) == densify._init_normals's output
+    recorded from the reference (tests/golden/depthmap_grid3x3.npz), bit for bit."""
+    z = np.load(os.path.join(GOLDEN, "depthmap_grid3x3.npz"))
+    for k in z["views"]:
+        got = cases.np_init_normals(z[f"depth_{k}"], z[f"ray_{k}"])
+        assert _same(got, z[f"normals_{k}"])
