@@ -1,19 +1,25 @@
 #!/usr/bin/env python
 """Throughput of the per-view depth-map path on one MI355X.
 
-Workload (default, the headline configuration): single-view PatchMatch at drone scale -- a
-2000x1500 reference view with 8 source views on a 2x4 grid (20 m baseline, 100 m altitude, f=1800)
-of the seeded synthetic terrain, seeded random slanted-plane start (depth U(80,105) m, normals
-within 30 deg of up), 11x11 window sampled every 2 px (36 samples), 8 red-black iterations, geometric
-term against the sources' rendered depth maps.  One step = one full PatchMatch refinement of the
-view (_dense.pm_refine semantics) from the seeded start, inputs resident in HBM.
+Workload (default, the headline configuration): the batched block -- 48 reference views of the
+seeded synthetic terrain from an 8 x 6 nadir grid at 90 m (70 % forward / 60 % side overlap),
+sizes drawn from 800x600, 1200x900, 1600x1200 and 2400x1800 (f = 0.9375 W), 4-12 nearest source
+views each; per view 192 fronto-parallel planes (11x11 NCC window, top-2) + the parabola, then 6
+PatchMatch iterations (9x9 window every 2 px) on every pixel with the neighbours' pass-1 depth maps
+as the geometric term.  One step = one replay of the HIP graph holding the whole block (one
+persistent sweep launch, the parabola, the PatchMatch passes of all views), inputs resident in HBM.
 
-Metric: Mpixel-hypotheses/s = H * W * 11 * iterations / step time (11 cost evaluations per pixel
-and iteration: current + 8 propagated + 2 random).  Also printed: p50/p99 step latency, the
-dominant kernel's roofline (HIP events around its launches), and the oracle (C restatement of the
-reference) timed on this host's cores on a bounded centre crop of the same workload.
+Metric: Mpixel-hypotheses/s = sum over views of H * W * (planes + 11 * iterations) / step time (one
+sweep pixel-hypothesis = one plane at one pixel over all sources; 11 PatchMatch cost evaluations
+per pixel and iteration: current + 8 propagated + 2 random).  Also printed: p50/p99 step latency,
+the dominant kernel's roofline (HIP events around its launches), the plain single-view GPU forms
+in the same run (fused per-view calls and the per-plane launch loop), and the oracle (C
+restatement of the reference) timed on this host's cores on centre crops of the same workload.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c2|c1|c3|c4|c5] [--no-cpu]
+Other workloads: c1 (640x480 sweep), c2 (2000x1500 PatchMatch), c4 (4000x3000 FP16/BF16), c5
+(served requests), fuse, sgm, prep.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c3|c1|c2|c4|c5|fuse|sgm|prep] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -33,6 +39,11 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpixel-hypotheses/s on 1 GPU; p50/p99 depth-map latency at fixed accuracy"
 FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
 HBM_PEAK = 8.0e12
+
+
+def progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def pm_flops_per_eval(S: int, nsamp: int) -> int:
@@ -185,7 +196,7 @@ def run_c2(args, torch):
                                "f=1800), 11x11 window step 2 (36 samples), 8 red-black iterations, geo term",
                    "H": H, "W": W, "sources": S, "iterations": c["iters"], "pixel_hypotheses_per_step": evals},
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
-        "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)", "achieved": achieved / 1e12,
+        "roofline": {"bound": "valu", "engine": "VALU FP64 (no MFMA: gather-and-reduce)", "achieved": achieved / 1e12,
                      "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
                      "traffic": profiled_traffic("pm_eval_c2.md"), "kernel": "pm_eval_kernel",
                      "kernel_avg_ms": eval_avg * 1e3, "launches": eval_n,
@@ -292,7 +303,7 @@ def run_c1(args, torch):
         "config": {"workload": "C1 plane sweep 640x480, 4 sources, 128 fronto-parallel planes 60-85 m, 5x5 NCC, "
                                "top-2, WTA", "H": H, "W": W, "sources": S, "planes": D},
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
-        "roofline": {"bound": "mfma", "engine": "VALU FP64", "achieved": flops / avg / 1e12 if avg else 0,
+        "roofline": {"bound": "valu", "engine": "VALU FP64", "achieved": flops / avg / 1e12 if avg else 0,
                      "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": (flops / avg / FP64_PEAK) if avg else 0,
                      "traffic": profiled_traffic("sweep_c1.md"), "kernel": "sweep_kernel", "kernel_avg_ms": avg * 1e3},
         "accuracy": {"valid_frac": float(valid.float().mean()),
@@ -345,6 +356,7 @@ def run_c3(args, torch):
     from py_ortho_amd import _lib, autotune, block
     lib = _lib.lib()
     t_scene = time.perf_counter()
+    progress("c3: rendering the 48-view block")
     sv = _survey_c3(torch, args.scene_cache, args.scale)
     torch.cuda.synchronize()
     t_scene = time.perf_counter() - t_scene
@@ -354,6 +366,7 @@ def run_c3(args, torch):
     o = blk.opt
     sw_ph, pm_ph = blk.pixel_hypotheses()
     t_tune = time.perf_counter()
+    progress("c3: autotuning tile shapes per size class")
     tune, table = autotune.tune_block(blk)
     t_tune = time.perf_counter() - t_tune
     blk.plan(tune)
@@ -371,6 +384,7 @@ def run_c3(args, torch):
     lib.om_profile_read(None, None, None)
     # the timed steps: graph replays of the whole block (sweep, parabola, PatchMatch of all views)
     blk.capture()
+    progress(f"c3: {args.warmup} warm-up + {args.steps} timed graph replays")
     for _ in range(args.warmup):
         blk.replay()
     torch.cuda.synchronize()
@@ -388,6 +402,7 @@ def run_c3(args, torch):
     truths = [v.depth for v in sv.views]
     acc = {"sweep": _depth_accuracy(raw_b, truths), "patchmatch": _depth_accuracy(dep_b, truths)}
     # the plain per-view form: fused single-view sweep + parabola + single-view PatchMatch per view
+    progress("c3: plain per-view calls")
     torch.cuda.synchronize()
     lib.om_profile_enable(1)
     ts = time.perf_counter()
@@ -401,7 +416,8 @@ def run_c3(args, torch):
         all(bool(torch.equal(a.view(torch.int32), b.view(torch.int32))) for a, b in zip(dep_b, blk.depth))
     plain = {"fused_per_view_ms": single_ms, "speedup_vs_fused_per_view": single_ms / ms,
              "fused_per_view_kernel_ms": single_parts}
-    if args.plain_per_hyp:
+    if not args.no_plain_per_hyp:
+        progress("c3: per-plane launch loop (plain single-view kernel shape)")
         torch.cuda.synchronize()
         ts = time.perf_counter()
         blk.run_single(per_hypothesis=True)
@@ -430,7 +446,7 @@ def run_c3(args, torch):
                                                  for (h, w), d in table.items()},
                    "autotune_s": t_tune, "scene_s": t_scene},
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
-        "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
+        "roofline": {"bound": "valu", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_batch_kernel",
                      "kernel_avg_ms": sw_avg * 1e3, "launches": sw_n, "flops_per_step": flops,
@@ -439,6 +455,7 @@ def run_c3(args, torch):
         "accuracy": acc,
     }
     if not args.no_cpu:
+        progress("c3: CPU baseline (oracle on centre crops)")
         res["cpu_baseline"] = cpu_baseline_c3(blk, sv, torch, args)
     return res
 
@@ -488,11 +505,13 @@ def cpu_baseline_c3(blk, sv, torch, args):
                         o.pm_geo_weight, o.geo_tol, iters, sv.dmin, sv.dmax, 1)
         return time.perf_counter() - t
 
-    target = 1.5                                     # seconds of work per core for each part
+    # crops: 512x512 for the sweep and 256x256 for PatchMatch, each shrunk (not below 128) only
+    # when the one-core estimate exceeds `budget` seconds
+    budget = 15.0
     sw_px = sweep(16, 8, 1) / (256 * 8)              # seconds per pixel-plane, one core
-    side_s = int(max(16, min(512, math.sqrt(target / (sw_px * o.num_depths)))))
+    side_s = int(max(128, min(512, min(H, W), math.sqrt(budget / (sw_px * o.num_depths)))))
     pm_px = pm(16, 1, 1) / 256                       # seconds per pixel-iteration, one core
-    side_p = int(max(16, min(512, math.sqrt(target / (pm_px * o.pm_iters)))))
+    side_p = int(max(128, min(256, min(H, W), math.sqrt(budget / (pm_px * o.pm_iters)))))
     ws = sweep(side_s, o.num_depths, cores)
     wp = pm(side_p, o.pm_iters, cores)
     per_px = ws / (side_s * side_s) + wp / (side_p * side_p)   # wall seconds per pixel for `cores` copies
@@ -500,7 +519,10 @@ def cpu_baseline_c3(blk, sv, torch, args):
             "cores": cores, "kind": "port",
             "sample": f"{cores} concurrent copies of the oracle sweep ({o.num_depths} planes, {side_s}x{side_s} "
                       f"centre crop) and pm_refine ({o.pm_iters} iterations, {side_p}x{side_p} crop) of view {k} "
-                      f"({W}x{H}, S={S_list[k]}); {ws:.1f} s + {wp:.1f} s wall"}
+                      f"({W}x{H}, S={S_list[k]}); {ws:.1f} s + {wp:.1f} s wall",
+            "calibration": "the oracle's sweep restatement runs at 0.85x the reference's own compiled _sweep.h "
+                           "(bit-identical, 640x480 view, 128 planes, 4 sources, one core of this build's "
+                           "container)"}
 
 
 def block_geo(blk, j, k):
@@ -653,7 +675,7 @@ def run_c5(args, torch):
         "plain_per_request": {"requests": n_plain, "mpix_hyp_s": plain, "speedup": served / plain,
                               "served_equals_plain": bool(same)},
         "accuracy": {"frac_within_1pct": float((rel < 0.01).mean()), "median_rel_err": float(np.median(rel))},
-        "roofline": {"bound": "mfma", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
+        "roofline": {"bound": "valu", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
                      "achieved": flops / (step_ms / 1e3) / 1e12 if step_ms else 0.0, "peak": FP64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": flops / (step_ms / 1e3) / FP64_PEAK if step_ms else 0.0,
                      "traffic": None, "kernel": "sweep_batch_kernel (om_sweep_multi steps)",
@@ -803,7 +825,7 @@ def run_c4(args, torch):
                           "pm_depth_within_0.2pct": float((rel_pm < 2e-3).float().mean())},
         "accuracy_vs_truth": {"pm_median_rel_err": float((((pm_lp - truth).abs() / truth)[both]).median()),
                               "pm_fp64_median_rel_err": float((((pm64 - truth).abs() / truth)[both]).median())},
-        "roofline": {"bound": "mfma", "engine": "VALU FP32 (no MFMA: gather-and-reduce)",
+        "roofline": {"bound": "valu", "engine": "VALU FP32 (no MFMA: gather-and-reduce)",
                      "achieved": flops / sw_avg / 1e12 if sw_avg else 0.0, "peak": 157.3, "unit": "TFLOP/s",
                      "frac": flops / sw_avg / 157.3e12 if sw_avg else 0.0, "traffic": None, "kernel": "sweep_lp_kernel",
                      "kernel_avg_ms": sw_avg * 1e3, "pm_propagate_ms_per_step": pr_ms / max(args.steps, 1)},
@@ -949,7 +971,7 @@ def run_fuse(args, torch):
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
         "points": npts, "candidates": ncand, "points_per_s": npts / (ms / 1e3),
         "consistency_ms_all_views": cons_s * 1e3,
-        "roofline": {"bound": "mfma", "engine": "VALU FP64", "achieved": flops / ev_s / 1e12 if ev_s else 0,
+        "roofline": {"bound": "valu", "engine": "VALU FP64", "achieved": flops / ev_s / 1e12 if ev_s else 0,
                      "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": (flops / ev_s / FP64_PEAK) if ev_s else 0,
                      "traffic": profiled_traffic("fuse_eval_block.md"), "kernel": "fuse_eval_kernel",
                      "kernel_avg_ms": ev_ms / max(ev_n, 1), "launches_per_step": ev_n / max(args.steps, 1)},
@@ -1158,7 +1180,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "fuse", "sgm", "prep"], default="c2")
+    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fuse", "sgm", "prep"], default="c3")
     ap.add_argument("--sgm-tiles", type=int, default=256, help="sgm: tiles aggregated in one call")
     ap.add_argument("--fmt", choices=["f16", "bf16"], default="f16", help="C4: image storage format")
     ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
@@ -1169,8 +1191,8 @@ def main(argv=None):
     ap.add_argument("--active-mpix", type=float, default=48.0,
                     help="C5: megapixels of admitted requests in flight (latency under overload)")
     ap.add_argument("--scale", type=float, default=1.0, help="C3, fuse: image size scale (1 = the full block)")
-    ap.add_argument("--plain-per-hyp", action="store_true",
-                    help="C3: also time the per-view form with one sweep launch per plane")
+    ap.add_argument("--no-plain-per-hyp", action="store_true",
+                    help="C3: skip timing the per-view form with one sweep launch per plane")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = all cores)")
     ap.add_argument("--scene-cache", default=None, help="npz file holding the rendered scene (created if absent)")
