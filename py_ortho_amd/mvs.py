@@ -184,9 +184,8 @@ def depth_map(k, nbrs, rec, views: Callable, opt: DepthOptions, device="cuda"):
 
     nd = plan.num_depths(rec, k, nbrs, rng, pyr_c[L].f, opt.num_depths)
     shape = tuple(pyr_r[L].shape)
-    inv_lo, inv_hi = plan.depth_band(rec, k, pyr_c[L], shape, rng, block=max(8, shape[1] // 16))
-    step_np = ((inv_hi - inv_lo) / max(nd - 1, 1)).astype(np.float32)
-    base, step = _dev(inv_lo, device), _dev(step_np, device)
+    base, _, step = plan.depth_band(rec, k, pyr_c[L], shape, rng, block=max(8, shape[1] // 16), nd=nd,
+                                    device=device)
     d, _, _ = sweep(pyr_r[L], pyr_c[L], pyr_s[L], pyr_cs[L], {"mode": "linear", "base": base, "step": step, "n": nd},
                     opt, rays[L])
     dstep = (step / 2).contiguous()
