@@ -21,6 +21,7 @@
  *   om_fuse               densify._fuse / _fuse_loop        orthomosaic/densify.py:676-762
  *   om_sgm                _mvs.sgm                          orthomosaic/_mvs.pyx:98-250
  *   om_depth_view         _fast.depth_view                  orthomosaic/_fast.pyx:577-616
+ *   om_depth_band         densify._depth_band + linear step orthomosaic/densify.py:233-274, :409-411
  *   om_f32_to_f16 / om_f16_to_f32 / om_filter_depth  the depth-map file set  densify.py:590-592, :613-614, :641
  */
 #ifndef OM355_H
@@ -140,6 +141,18 @@ size_t om_box_workspace_bytes(int32_t H, int32_t W, int32_t r);
 int om_ref_stats(const float *ref, int32_t H, int32_t W, int32_t r, float *mu, float *sd, void *workspace,
                  size_t workspace_bytes, void *stream);
 int om_half(const float *g, int32_t H, int32_t W, float *out, void *stream);
+/* densify._depth_band (densify.py:233-274) on the device: uvz (n, 3) float64 = the view's own tie
+ * points in front of the coarse camera, projected (u, v) with their camera-axis depth z > 0;
+ * n_obs = the view's observation count (below 50: the range [lo_g, hi_g] everywhere).  Per block of
+ * block x block pixels with >= 5 points: the depth span over the 3x3 block neighbourhood widened by
+ * margin + 5 % of it, clipped to [lo_g, hi_g]; other blocks [lo_g, hi_g].  Writes inv_lo = f32(1 / hi)
+ * and, when not NULL, inv_hi = f32(1 / max(lo, 1e-6)) and step = (inv_hi - inv_lo) / max(nd - 1, 1)
+ * in float32 (the coarse linear generator, densify.py:411), all (H, W).  Workspace:
+ * om_depth_band_workspace_bytes(H, W, block) bytes of device memory (unused when n_obs < 50). */
+size_t om_depth_band_workspace_bytes(int32_t H, int32_t W, int32_t block);
+int om_depth_band(const double *uvz, int64_t n, int32_t n_obs, int32_t H, int32_t W, int32_t block, double lo_g,
+                  double hi_g, double margin, int32_t nd, float *inv_lo, float *inv_hi, float *step, void *workspace,
+                  size_t workspace_bytes, void *stream);
 /* densify._rays_world (densify.py:288-297): (3, H, W) float32 world rays, camera-axis depth 1;
  * R: host row-major world->camera rotation. */
 int om_rays_world(int32_t H, int32_t W, const double *R, double cx, double cy, double f, double k1, double k2,
