@@ -102,6 +102,12 @@ class ROCmBackend:
         from . import dense
         return dense.sweep_hypothesis(*args, **kw)
 
+    def densify(self, ar, rec, gains, biases, out_dir, opt, **kw):
+        """densify.densify (densify.py:499-510) on this device: depth maps, repair and fusion in HBM
+        -> DenseCloud.  The reference's densify hands over here when ar.backend is this backend."""
+        from . import densify
+        return densify.densify(ar, rec, gains, biases, out_dir, opt, device=self.device, **kw)
+
     # ------------------------------------------------------------------ height-sweep array ops
     def box(self, a, r):
         from . import ops

@@ -1,0 +1,216 @@
+"""Densification on the MI355X: the drop-in for pyOrthomosaic's `densify.densify`.
+
+`densify(ar, rec, gains, biases, out_dir, opt)` takes the reference's arguments and returns its
+`DenseCloud` (densify.py:499-662), with every array after image decode resident in HBM:
+
+1. views: each decoded uint8 image goes to the device once (3 bytes a pixel) and becomes the
+   matching grey and the fusion colours there (`om_depth_view`);
+2. pass 1: the depth map of every view (`mvs.depth_map`: pyramid, coarse band, sweep, two
+   refinement levels); the score is kept as float16 and widened again, as the reference stores it
+   (densify.py:590-592, :613);
+3. pass 2: per view the geometric consistency count against the neighbours' pass-1 maps
+   (`om_consistency`), the PatchMatch inputs -- the pixels to repair, normals, the neighbours' maps
+   as source depths, the median relative tolerance, the depth range or the map's own span, the
+   view's seed -- and `pm_refine`, then the final filter (densify.py:609-641);
+4. fusion: the reference's sequential fusion over the final maps (`om_fuse`, densify.py:656-666).
+
+Nothing goes through files: the `.npy` depth-map set is written only when `opt.keep_depthmaps`
+is set, with the reference's names and bytes.  Image decode stays on the host: `load_rgb(frame,
+scale)` and `load_mask(path, w, h)` are the reference's imageio.load_rgb and masks.load_mask unless
+other loaders are passed.
+
+Views are processed one after another on the device stream, each launch spanning the whole GPU.
+"""
+from __future__ import annotations
+
+import logging
+import os
+import shutil
+import time
+from typing import Callable, Dict, Optional
+
+import numpy as np
+import torch
+
+from . import dense, formats, fusion, mvs, plan, prep
+
+log = logging.getLogger(__name__)
+
+__all__ = ["densify", "DensifyRun"]
+
+
+def _default_loaders():
+    from orthomosaic.imageio import load_rgb  # pyOrthomosaic's decoders, when installed beside this package
+    from orthomosaic.masks import load_mask
+    return load_rgb, load_mask
+
+
+class DensifyRun:
+    """One densification of a block.  The stages can be called one by one (tests, benchmarks);
+    densify() runs them all."""
+
+    def __init__(self, ar, rec, gains: dict, biases: Optional[dict], opt, load_rgb: Optional[Callable] = None,
+                 load_mask: Optional[Callable] = None, device="cuda"):
+        self.ar, self.rec, self.opt = ar, rec, opt
+        self.gains, self.biases = gains or {}, biases
+        if load_rgb is None or load_mask is None:
+            d_rgb, d_mask = _default_loaders()
+            load_rgb, load_mask = load_rgb or d_rgb, load_mask or d_mask
+        self.load_rgb, self.load_mask = load_rgb, load_mask
+        self.device = torch.device(device)
+        self.N = len(rec.used)
+        self._cameras()
+        self._views: Dict[int, tuple] = {}
+        self.nbrs = plan.neighbours(rec, opt.neighbors)
+        self._settings()
+        self.raw: Dict[int, torch.Tensor] = {}
+        self.score16: Dict[int, torch.Tensor] = {}
+        self.low: Dict[int, torch.Tensor] = {}
+        self.final: Dict[int, torch.Tensor] = {}
+        self.stats = {"repaired": 0, "active": 0, "pixels": 0}
+
+    # ------------------------------------------------------------------ set-up (densify.py:517-586)
+    def _cameras(self):
+        rec, frames, opt = self.rec, self.ar.frames, self.opt
+        self.scales, self.cams = {}, {}
+        for k, i in enumerate(rec.used):
+            fr = frames[i]
+            long_side = max(fr.width, fr.height)
+            s = min(1.0, (opt.max_image_dim or long_side) / long_side)
+            w, h = int(round(fr.width * s)), int(round(fr.height * s))
+            it = rec.intr[rec.cam_group[k]]
+            full = mvs.Cam(rec.R[k], rec.C[k], it.f, it.k1, it.k2, it.cx, it.cy, fr.width, fr.height,
+                           getattr(it, "k3", 0.0))
+            self.cams[k] = full.scaled(w / fr.width, w, h)
+            self.scales[k] = s
+
+    def _settings(self):
+        rec, opt = self.rec, self.opt
+        tp = plan.tie_points(rec)
+        own_z = {}
+        for k in range(self.N):
+            idx = tp.own(k)
+            if len(idx):
+                own_z[k] = (rec.R[k] @ (rec.X[idx] - rec.C[k]).T)[2]
+        self.med_gsd = float(np.median([float(np.nanmedian(own_z[k])) / self.cams[k].f for k in sorted(own_z)]))
+        scale_med = float(np.median(list(self.scales.values())))
+        self.px_tol = opt.consistency_px or float(np.clip(2.0 * rec.rms_px * scale_med, 1.0, 3.0))
+        n_good = np.array([len(x) for x in self.nbrs])
+        self.min_views = opt.min_views or (3 if np.median(n_good) >= 3 else 2)
+        self.dmed = {k: float(np.median(z)) for k, z in own_z.items()}
+
+    def rel_tols(self, k: int) -> dict:
+        """Relative depth tolerance per neighbour: the depth change that moves a point by px_tol
+        pixels between the two views, in [0.01, 0.05] (densify.py:574-582)."""
+        C, out = self.rec.C, {}
+        for j in self.nbrs[k]:
+            B = float(np.linalg.norm(C[k] - C[j]))
+            out[j] = (self.opt.consistency_depth or
+                      float(np.clip(self.px_tol * self.dmed.get(k, 1.0) / max(self.cams[k].f * B, 1e-9), 0.01, 0.05)))
+        return out
+
+    # ------------------------------------------------------------------ views
+    def view(self, k: int):
+        """(grey float32 (h, w) NaN = masked, rgb8 uint8 (h, w, 3), camera) of view k on the device,
+        decoded and prepared once (densify.py:528-535)."""
+        v = self._views.get(k)
+        if v is None:
+            i = self.rec.used[k]
+            rgb = np.ascontiguousarray(self.load_rgb(self.ar.frames[i], self.scales[k]), np.uint8)
+            mask = self.load_mask(self.ar.frames[i].path, rgb.shape[1], rgb.shape[0])
+            b = None if self.biases is None else self.biases.get(i)
+            grey, rgb8, _ = prep.prepare_view(rgb, self.gains.get(i), b, mask, levels=1, device=self.device)
+            v = (grey, rgb8, self.cams[k])
+            self._views[k] = v
+        return v
+
+    # ------------------------------------------------------------------ pass 1 (densify.py:587-603)
+    def pass1(self):
+        for k in range(self.N):
+            r = mvs.depth_map(k, self.nbrs[k], self.rec, self.view, self.opt, device=self.device)
+            if r is None:
+                continue
+            depth, score, low = r
+            self.raw[k] = depth.contiguous()
+            self.score16[k] = formats.to_f16(score)
+            self.low[k] = low
+
+    # ------------------------------------------------------------------ pass 2 (densify.py:609-641)
+    def repair(self, k: int) -> torch.Tensor:
+        """View k's final depth map: PatchMatch over its inconsistent pixels (when opt.patchmatch),
+        then NaN where the score is below min_ncc or the window is flat."""
+        opt, rec = self.opt, self.rec
+        D = self.raw[k].clone()
+        S = formats.to_f32(self.score16[k])
+        if opt.patchmatch:
+            tol = self.rel_tols(k)
+            cnt = fusion.consistency(k, D, self.nbrs[k], self.raw, self.cams, self.px_tol, tol)
+            grey, _, cr = self.view(k)
+            active = (torch.isfinite(grey) & ((cnt < self.min_views - 1) | (S < opt.min_ncc) | ~torch.isfinite(D)))
+            active = active.to(torch.uint8).contiguous()
+            src_ids = [j for j in self.nbrs[k] if j in self.raw]
+            n_active = int(active.sum())
+            if n_active and src_ids:
+                ray = mvs.rays_world(tuple(D.shape), cr, self.device)
+                normal = mvs.init_normals(D, ray)
+                srcs = [self.view(j)[0] for j in src_ids]
+                par = np.array([np.r_[self.cams[j].R.ravel(), self.cams[j].C, self.cams[j].f, self.cams[j].k1,
+                                      self.cams[j].k2, self.cams[j].k3, self.cams[j].cx, self.cams[j].cy]
+                                for j in src_ids], np.float64)
+                sdeps = [self.raw[j] for j in src_ids]
+                rng = plan.depth_range(rec, k)
+                if rng is None:                      # the map's own span (np.nanmin / np.nanmax)
+                    fin = D[torch.isfinite(D)]
+                    rng = (float(fin.min()), float(fin.max())) if fin.numel() else (float("nan"), float("nan"))
+                seed = k * 2654435761 % 2 ** 32
+                ch = dense.pm_refine(grey, ray, np.asarray(cr.C, np.float64), srcs, sdeps, par, D, normal, S, active,
+                                     opt.pm_window, opt.pm_step, opt.top_k, opt.pm_geo_weight,
+                                     float(np.median(list(tol.values()))), opt.pm_iters, rng[0], rng[1], seed)
+                self.stats["repaired"] += int(ch)
+                self.stats["active"] += n_active
+            self.stats["pixels"] += int(torch.isfinite(D).sum())
+        # D[(S < min_ncc) | low] = NaN with the float32 score (densify.py:638)
+        bad = (S < opt.min_ncc) | self.low[k]
+        return torch.where(bad, torch.full_like(D, float("nan")), D).contiguous()
+
+    def pass2(self):
+        for k in sorted(self.raw):
+            self.final[k] = self.repair(k)
+
+    # ------------------------------------------------------------------ fusion (densify.py:652-666)
+    def fuse(self) -> formats.DenseCloud:
+        have = sorted(self.final)
+        pts, cols, nv, owner = [], [], [], []
+        if have:
+            rgb = {k: self.view(k)[1] for k in have}
+            fusion.fuse(have, self.final, None, self.cams, self.nbrs, set(have), self.rel_tols, self.opt,
+                        self.px_tol, self.min_views, rgb, pts, cols, nv, owner)
+        return formats.dense_cloud(pts, cols, nv, owner, self.med_gsd * max(1, self.opt.point_stride))
+
+    def save(self, wd: str):
+        """The depth-map files of densify's two passes (densify.py:590-592, :639)."""
+        os.makedirs(wd, exist_ok=True)
+        for k in sorted(self.raw):
+            np.save(os.path.join(wd, f"p{k:05d}.npy"), self.raw[k].cpu().numpy())
+            np.save(os.path.join(wd, f"s{k:05d}.npy"), self.score16[k].cpu().numpy())
+            np.save(os.path.join(wd, f"t{k:05d}.npy"), self.low[k].to(torch.bool).cpu().numpy())
+        for k, d in sorted(self.final.items()):
+            formats.save_final(wd, k, d)
+
+
+def densify(ar, rec, gains: dict, biases: Optional[dict], out_dir: str, opt, load_rgb: Optional[Callable] = None,
+            load_mask: Optional[Callable] = None, device="cuda") -> formats.DenseCloud:
+    """densify.densify (densify.py:499-510) on the device -> DenseCloud.  <out_dir>/depthmaps holds
+    the reference's depth-map files afterwards only when opt.keep_depthmaps is set."""
+    wd = os.path.join(out_dir, "depthmaps")
+    shutil.rmtree(wd, ignore_errors=True)
+    t0 = time.time()
+    run = DensifyRun(ar, rec, gains, biases, opt, load_rgb, load_mask, device)
+    run.pass1()
+    log.info("Depth maps done in %.1fs (%d/%d images)", time.time() - t0, len(run.raw), run.N)
+    run.pass2()
+    if opt.keep_depthmaps:
+        run.save(wd)
+    cloud = run.fuse()
+    log.info("Dense cloud: %d points fused in %.1fs", len(cloud.xyz), time.time() - t0)
+    return cloud
