@@ -10,6 +10,7 @@
  * Reference interfaces replaced (pyOrthomosaic 0.9.3):
  *   om_sweep_hypothesis   _dense.sweep_hypothesis          orthomosaic/_dense.pyx:740-838
  *   om_sweep              the hypothesis loop of _sweep      orthomosaic/densify.py:336-342
+ *   om_sweep_rows         ncc_warp + combine (n > 16 path)   orthomosaic/_dense.pyx:13-144, densify.py:343-353
  *   om_sweep_finalize     the parabola of _sweep             orthomosaic/densify.py:354-361
  *   om_pm_refine          _dense.pm_refine                  orthomosaic/_dense.pyx:566-666
  *   om_box_zero_mean      _fast.box_zero_mean (densify._box) orthomosaic/_fast.pyx:661-713
@@ -96,6 +97,15 @@ int om_device_info(char *name, int32_t name_len, int32_t *cus, int32_t *clock_kh
 /* fused plane sweep: hypotheses [i0, i0+nh) of one view in one launch (+ a merge when chunked) */
 size_t om_sweep_workspace_bytes(const om_sweep_desc *d);
 int om_sweep(const om_sweep_desc *d, void *stream);
+/* The reference's unfused sweep (_dense.ncc_warp per source + _dense.combine, _dense.pyx:13-144,
+ * the path densify._sweep takes above 16 sources, densify.py:343-353): horizontal running sums along
+ * whole rows and vertical ones down whole columns in the reference's order, so every result equals
+ * the reference bit for bit for any input.  Any source count (1..256), window radius r >= 0 and
+ * top_k >= 1; same descriptor as om_sweep, with d->chunks = hypotheses per pass (0 = choose) and
+ * a required workspace of om_sweep_rows_workspace_bytes(d) bytes (the pass's window sums and
+ * scores, 20 bytes per pixel, source and hypothesis). */
+size_t om_sweep_rows_workspace_bytes(const om_sweep_desc *d);
+int om_sweep_rows(const om_sweep_desc *d, void *stream);
 /* om_sweep that allocates nothing.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same descriptor and results as om_sweep; d->workspace is ignored and the hypothesis

@@ -301,15 +301,17 @@ ORC_EXPORT int orc_ncc_warp(const float *ref, const float *mu, const float *sd, 
     return ORC_OK;
 }
 
-/* Unfused combine: _dense.pyx:113-144.  S: (n,H,W). */
+/* Unfused combine: _dense.pyx:113-144.  S: (n,H,W).  The reference sorts into `float buf[16]`,
+ * which its n > 16 caller (densify.py:343-353) overruns; this restatement sorts all n sources (up
+ * to 256), as the loop is written. */
 ORC_EXPORT int orc_combine(const float *S, int n, int64_t H, int64_t W, int top_k, int i, float *best, float *prev,
                            float *spb, float *snb, int32_t *idx)
 {
-    if (n < 1 || n > 16)
+    if (n < 1 || n > 256)
         return ORC_EINVAL;
     const int64_t HW = H * W;
     const int k = top_k < n ? top_k : n;
-    float buf[16];
+    float buf[256];
     for (int64_t p = 0; p < HW; p++) {
         for (int a = 0; a < n; a++)
             buf[a] = S[a * HW + p];

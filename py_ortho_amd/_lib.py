@@ -75,6 +75,8 @@ SIGNATURES = {
     "om_depth_band_workspace_bytes": (c_size_t, [c_int32, c_int32, c_int32]),
     "om_depth_band": (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32, c_int32, c_double, c_double, c_double,
                                 c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "om_sweep_rows_workspace_bytes": (c_size_t, [ctypes.POINTER(OmSweepDesc)]),
+    "om_sweep_rows": (c_int32, [ctypes.POINTER(OmSweepDesc), c_void_p]),
     "om_sweep_noalloc_max_bytes": (c_size_t, []),
     "om_sweep_noalloc": (c_int32, [ctypes.POINTER(OmSweepDesc), c_void_p]),
     "om_sweep_hypothesis": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,

@@ -591,25 +591,29 @@ __global__ void sweep_batch_reset_kernel(const SweepK *views, const int64_t *px0
     P.idx[p] = -1;
 }
 
-static int build_params(const om_sweep_desc *d, SweepK &P)
+// fused: the limits of the fused kernels (1..16 sources, r <= kMaxR, min(top_k, n) <= 4); without
+// them (the parabola, which reads only the winner state) any source count of om_sweep_rows is taken
+// and the sources are not copied
+static int build_params(const om_sweep_desc *d, SweepK &P, bool fused = true)
 {
     if (!d)
         return fail(OM_EINVAL, "om_sweep: null descriptor");
-    if (d->n_src < 1 || d->n_src > 16)
-        return fail(OM_ENSRC, "sweep needs 1..16 sources (got %d)", d->n_src);
+    if (d->n_src < 1 || d->n_src > (fused ? 16 : 256))
+        return fail(OM_ENSRC, "sweep needs 1..%d sources (got %d)", fused ? 16 : 256, d->n_src);
     if (d->r < 0)
         return fail(OM_EINVAL, "negative window radius");
     if (d->H < 0 || d->W < 0 || d->nh < 0)
         return fail(OM_ESHAPE, "negative size");
     if (!tap_size_ok(d->H, d->W))
         return fail(OM_ESHAPE, "sweep: reference images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported");
-    if (d->r > kMaxR)
-        return fail(OM_EUNSUP, "fused sweep supports window radius <= %d (got %d)", kMaxR, d->r);
+    if (fused && d->r > kMaxR)
+        return fail(OM_EUNSUP, "fused sweep supports window radius <= %d (got %d; om_sweep_rows takes any)", kMaxR,
+                    d->r);
     if (d->top_k < 1)
         return fail(OM_EINVAL, "top_k must be >= 1");
     const int k = d->top_k < d->n_src ? d->top_k : d->n_src;
-    if (k > 4)
-        return fail(OM_EUNSUP, "fused sweep supports min(top_k, n) <= 4 (got %d)", k);
+    if (fused && k > 4)
+        return fail(OM_EUNSUP, "fused sweep supports min(top_k, n) <= 4 (got %d; om_sweep_rows takes any)", k);
     if (d->hyp_mode < OM_HYP_MAPS || d->hyp_mode > OM_HYP_CONST)
         return fail(OM_EINVAL, "unknown hypothesis generator %d", d->hyp_mode);
     if (!d->src)
@@ -637,7 +641,7 @@ static int build_params(const om_sweep_desc *d, SweepK &P)
     P.snb = d->s_next_best;
     P.idx = d->idx;
     P.pinhole = 1;
-    for (int s = 0; s < d->n_src; ++s) {
+    for (int s = 0; s < (fused ? d->n_src : 0); ++s) {
         const om_src &o = d->src[s];
         if (!tap_size_ok(o.h, o.w))
             return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more, or with a side of 2^24 or more, are not supported", s);
@@ -834,7 +838,7 @@ extern "C" int om_sweep_hypothesis(const float *ref, const float *mu, const floa
 extern "C" int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *stream)
 {
     SweepK P;
-    int rc = build_params(d, P);
+    int rc = build_params(d, P, false);
     if (rc)
         return rc;
     if (d->i0 != 0)

@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from ._lib import OmSrc, OmSweepDesc, check
 
-__all__ = ["sweep_hypothesis", "sweep", "sweep_finalize", "pm_refine", "box_zero_mean", "hyp_linear",
+__all__ = ["sweep_hypothesis", "sweep", "fused_ok", "sweep_finalize", "pm_refine", "box_zero_mean", "hyp_linear",
            "hyp_refine", "half", "ref_stats", "SweepState"]
 
 
@@ -107,6 +107,19 @@ def sweep_hypothesis(ref, mu_i, sd_i, srcs, ray, geo, inv, r, top_k, i, best, pr
         raise ValueError("sweep_hypothesis: negative window radius")
     if H == 0 or W == 0:
         return
+    if not fused_ok(n, r, top_k):
+        # the reference takes any r and top_k here: the row/column-streaming kernels, map i0 = i
+        st = SweepState.__new__(SweepState)
+        st.best, st.prev, st.s_prev_best, st.s_next_best, st.idx = best, prev, s_prev_best, s_next_best, idx
+        _need(inv, "inv")
+        d, keep = _sweep_desc(ref, mu_i, sd_i, srcs, ray, g, r, top_k, st, _lib.OM_HYP_MAPS, inv, None, 0, int(i), 1)
+        L = _lib.lib()
+        nbytes = L.om_sweep_rows_workspace_bytes(ctypes.byref(d))
+        ws = _workspace(nbytes, ref.device)
+        d.workspace, d.workspace_bytes = ws.data_ptr(), nbytes
+        check(L.om_sweep_rows(ctypes.byref(d), _stream()), "sweep_hypothesis")
+        del keep, ws
+        return
     ptrs = (ctypes.c_void_p * n)(*[_need(s, f"srcs[{k}]") for k, s in enumerate(srcs)])
     hw = (ctypes.c_int32 * (2 * n))(*[v for s in srcs for v in (int(s.shape[0]), int(s.shape[1]))])
     gd = (ctypes.c_double * (18 * n))(*g.ravel().tolist())
@@ -121,8 +134,8 @@ def sweep_hypothesis(ref, mu_i, sd_i, srcs, ray, geo, inv, r, top_k, i, best, pr
 def _sweep_desc(ref, mu, sd, srcs, ray, geo, r, top_k, state: SweepState, mode, hyp_a, hyp_b=None,
                 hyp_offset=0, i0=0, nh=0, chunks=0, tile_h=0, src_dtype=torch.float32):
     n = len(srcs)
-    if n < 1 or n > 16:
-        raise ValueError("sweep needs 1..16 sources")
+    if n < 1 or n > MAX_SOURCES:
+        raise ValueError(f"sweep needs 1..{MAX_SOURCES} sources")
     g = _geo(geo, n)
     H, W = int(ref.shape[0]), int(ref.shape[1])
     d = OmSweepDesc()
@@ -171,20 +184,48 @@ def _hyp_inputs(hyp: dict, H: int, W: int):
     raise ValueError(f"unknown hypothesis mode {m!r}")
 
 
+MAX_SOURCES = 256      # sources of om_sweep_rows; the fused kernels take 16
+FUSED_MAX_R = 7        # window radius of the fused kernels
+FUSED_MAX_K = 4        # min(top_k, sources) of the fused kernels
+
+
+def fused_ok(n: int, r: int, top_k: int) -> bool:
+    """Whether the fused kernels take this sweep (else om_sweep_rows does)."""
+    return 1 <= n <= 16 and 0 <= r <= FUSED_MAX_R and 1 <= min(top_k, n) <= FUSED_MAX_K
+
+
 def sweep(ref, mu, sd, srcs, ray, geo, hyp: dict, r: int, top_k: int, state: Optional[SweepState] = None,
-          i0: int = 0, nh: Optional[int] = None, chunks: int = 0, tile_h: int = 0) -> SweepState:
-    """Hypotheses [i0, i0+nh) of one view in one fused launch: equal to calling sweep_hypothesis
-    for i = i0 .. i0+nh-1 in order (densify.py:339-342) on the same state."""
+          i0: int = 0, nh: Optional[int] = None, chunks: int = 0, tile_h: int = 0, exact: bool = False) -> SweepState:
+    """Hypotheses [i0, i0+nh) of one view: equal to calling sweep_hypothesis for i = i0 .. i0+nh-1
+    in order (densify.py:339-342) on the same state.
+
+    The fused kernel (one launch) takes 1..16 sources, r <= 7 and min(top_k, n) <= 4; its window sums
+    equal the reference's whenever no FP64 partial sum rounds.  Otherwise, or with exact=True, the
+    row/column-streaming kernels (om_sweep_rows) run the reference's ncc_warp + combine order, bit
+    for bit for any input (densify.py:343-353); `chunks` is then the hypotheses per pass."""
     H, W = int(ref.shape[0]), int(ref.shape[1])
     mode, a, b, off, n_all = _hyp_inputs(hyp, H, W)
     if state is None:
         state = SweepState(H, W, ref.device)
     if nh is None:
         nh = n_all - i0
-    d, keep = _sweep_desc(ref, mu, sd, srcs, ray, geo, r, top_k, state, mode, a, b, off, i0, nh, chunks, tile_h)
+    if r < 0:
+        raise ValueError("sweep: negative window radius")
+    rows = exact or not fused_ok(len(srcs), r, top_k)
+    d, keep = _sweep_desc(ref, mu, sd, srcs, ray, geo, r, top_k, state, mode, a, b, off, i0, nh, chunks,
+                          0 if rows else tile_h)
     if mode in (_lib.OM_HYP_MAPS, _lib.OM_HYP_CONST) and i0 != 0:
         raise ValueError("maps / const hypotheses are indexed from i0 = 0 in this call")
     L = _lib.lib()
+    if rows:
+        nbytes = L.om_sweep_rows_workspace_bytes(ctypes.byref(d))
+        if nbytes == 0 and H * W and nh:
+            check(L.om_sweep_rows(ctypes.byref(d), _stream()), "sweep_rows")   # raises with the reason
+        ws = _workspace(nbytes, ref.device)
+        d.workspace, d.workspace_bytes = ws.data_ptr(), nbytes
+        check(L.om_sweep_rows(ctypes.byref(d), _stream()), "sweep_rows")
+        del keep, ws
+        return state
     nbytes = L.om_sweep_workspace_bytes(ctypes.byref(d))
     ws = None
     if nbytes:

@@ -50,8 +50,9 @@ class DensifyRun:
     densify() runs them all."""
 
     def __init__(self, ar, rec, gains: dict, biases: Optional[dict], opt, load_rgb: Optional[Callable] = None,
-                 load_mask: Optional[Callable] = None, device="cuda"):
+                 load_mask: Optional[Callable] = None, device="cuda", exact: bool = False):
         self.ar, self.rec, self.opt = ar, rec, opt
+        self.exact = bool(exact)
         self.gains, self.biases = gains or {}, biases
         if load_rgb is None or load_mask is None:
             d_rgb, d_mask = _default_loaders()
@@ -127,7 +128,7 @@ class DensifyRun:
     # ------------------------------------------------------------------ pass 1 (densify.py:587-603)
     def pass1(self):
         for k in range(self.N):
-            r = mvs.depth_map(k, self.nbrs[k], self.rec, self.view, self.opt, device=self.device)
+            r = mvs.depth_map(k, self.nbrs[k], self.rec, self.view, self.opt, device=self.device, exact=self.exact)
             if r is None:
                 continue
             depth, score, low = r
@@ -199,13 +200,16 @@ class DensifyRun:
 
 
 def densify(ar, rec, gains: dict, biases: Optional[dict], out_dir: str, opt, load_rgb: Optional[Callable] = None,
-            load_mask: Optional[Callable] = None, device="cuda") -> formats.DenseCloud:
+            load_mask: Optional[Callable] = None, device="cuda", exact: bool = False) -> formats.DenseCloud:
     """densify.densify (densify.py:499-510) on the device -> DenseCloud.  <out_dir>/depthmaps holds
-    the reference's depth-map files afterwards only when opt.keep_depthmaps is set."""
+    the reference's depth-map files afterwards only when opt.keep_depthmaps is set.  exact: the
+    sweeps in the reference's summation order (om_sweep_rows), so the cloud is the reference's bit
+    for bit for any input; otherwise the fused sweep, equal to it except where an FP64 window sum
+    rounds (scores within 1e-4)."""
     wd = os.path.join(out_dir, "depthmaps")
     shutil.rmtree(wd, ignore_errors=True)
     t0 = time.time()
-    run = DensifyRun(ar, rec, gains, biases, opt, load_rgb, load_mask, device)
+    run = DensifyRun(ar, rec, gains, biases, opt, load_rgb, load_mask, device, exact)
     run.pass1()
     log.info("Depth maps done in %.1fs (%d/%d images)", time.time() - t0, len(run.raw), run.N)
     run.pass2()

@@ -115,10 +115,11 @@ def _geo(cam_r: Cam, cams_s: Sequence[Cam]) -> np.ndarray:
     return np.asarray(rows, np.float64)
 
 
-def sweep(ref, cam_r, srcs, cams_s, hyp: dict, opt: DepthOptions, ray=None):
+def sweep(ref, cam_r, srcs, cams_s, hyp: dict, opt: DepthOptions, ray=None, exact: bool = False):
     """densify._sweep with the hypotheses given by a generator description (see dense.sweep):
-    winner-take-all over all hypotheses in one fused launch, then the parabola.
-    -> (depth, best, sd) device float32 (H, W)."""
+    winner-take-all over all hypotheses in one fused launch (exact=True, more than 16 sources, a
+    window radius above 7 or a top-k above 4: the row/column-streaming kernels in the reference's
+    summation order), then the parabola.  -> (depth, best, sd) device float32 (H, W)."""
     r = int(opt.window)
     ref = _dev(ref)
     H, W = ref.shape
@@ -129,17 +130,17 @@ def sweep(ref, cam_r, srcs, cams_s, hyp: dict, opt: DepthOptions, ray=None):
     mu, sd = dense.ref_stats(ref, r)
     src = [_dev(s) for s in srcs]
     geo = _geo(cam_r, cams_s)
-    st = dense.sweep(ref, mu, sd, src, ray, geo, hyp, r, opt.top_k)
+    st = dense.sweep(ref, mu, sd, src, ray, geo, hyp, r, opt.top_k, exact=exact)
     depth = dense.sweep_finalize(ref, mu, sd, src, ray, geo, hyp, r, opt.top_k, st)
     return depth, st.best, sd
 
 
-def _sweep(ref, cam_r, srcs, cams_s, inv_list, opt: DepthOptions, ray=None):
+def _sweep(ref, cam_r, srcs, cams_s, inv_list, opt: DepthOptions, ray=None, exact: bool = False):
     """The reference signature (densify.py:316): `inv_list` is any sequence of (H, W) inverse-depth
     maps, uniformly spaced (numpy arrays, tensors, or a lazy sequence like densify._Hypotheses)."""
     ref_t = _dev(ref)
     maps = torch.stack([_dev(inv_list[i], ref_t.device) for i in range(len(inv_list))])
-    return sweep(ref_t, cam_r, srcs, cams_s, {"mode": "maps", "maps": maps}, opt, ray)
+    return sweep(ref_t, cam_r, srcs, cams_s, {"mode": "maps", "maps": maps}, opt, ray, exact)
 
 
 def _upsample2(a: torch.Tensor, H: int, W: int) -> torch.Tensor:
@@ -151,10 +152,11 @@ def _upsample2(a: torch.Tensor, H: int, W: int) -> torch.Tensor:
     return up.contiguous()
 
 
-def depth_map(k, nbrs, rec, views: Callable, opt: DepthOptions, device="cuda"):
+def depth_map(k, nbrs, rec, views: Callable, opt: DepthOptions, device="cuda", exact: bool = False):
     """Depth (camera-axis, metres), NCC score and low-texture flag of reference image k at matching
     scale (densify.py:373-437), on the device.  views(j) -> (grey (h, w) float32 NaN = masked,
-    rgb, camera) as in the reference.  Returns None when the image has no range or no neighbours."""
+    rgb, camera) as in the reference.  Returns None when the image has no range or no neighbours.
+    exact: every sweep in the reference's summation order (see sweep)."""
     rng = plan.depth_range(rec, k)
     if rng is None or not nbrs:
         return None
@@ -187,7 +189,7 @@ def depth_map(k, nbrs, rec, views: Callable, opt: DepthOptions, device="cuda"):
     base, _, step = plan.depth_band(rec, k, pyr_c[L], shape, rng, block=max(8, shape[1] // 16), nd=nd,
                                     device=device)
     d, _, _ = sweep(pyr_r[L], pyr_c[L], pyr_s[L], pyr_cs[L], {"mode": "linear", "base": base, "step": step, "n": nd},
-                    opt, rays[L])
+                    opt, rays[L], exact)
     dstep = (step / 2).contiguous()
     rs = int(opt.refine_steps)
     score = sd = None
@@ -197,7 +199,7 @@ def depth_map(k, nbrs, rec, views: Callable, opt: DepthOptions, device="cuda"):
         dstep = _upsample2(dstep, H, W)
         iu = (1.0 / up).contiguous()
         d, score, sd = sweep(pyr_r[lvl], pyr_c[lvl], pyr_s[lvl], pyr_cs[lvl],
-                             {"mode": "refine", "iu": iu, "ds": dstep, "steps": rs}, opt, rays[lvl])
+                             {"mode": "refine", "iu": iu, "ds": dstep, "steps": rs}, opt, rays[lvl], exact)
         dstep = (dstep / 2).contiguous()
     depth = d
     sd = sd * 64.0

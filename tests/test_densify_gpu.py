@@ -40,6 +40,11 @@ def _same(a, b):
 
 @pytest.mark.parametrize("name", list(SETTINGS))
 def test_densify_matches_reference(name, tmp_path):
+    """The default (fused sweep) densify against the reference under the parity bar: every depth-map
+    file equal to the reference's except at a few pixels (an FP64 window sum rounding differently
+    moves a score by ~1e-8, which can move a parabola step and what PatchMatch does from there), depth
+    within 0.2 % on >= 99 % of the pixels finite in both, and >= 98 % of the reference's points
+    present bit for bit in the cloud."""
     from py_ortho_amd import densify, mvs
     z = np.load(GOLDEN)
     ar, rec, gains, biases, load_rgb, load_mask = _inputs(z)
@@ -47,19 +52,30 @@ def test_densify_matches_reference(name, tmp_path):
     cloud = densify.densify(ar, rec, gains, biases, str(tmp_path), opt, load_rgb=load_rgb, load_mask=load_mask)
     torch.cuda.synchronize()
     wd = tmp_path / "depthmaps"
-    files = sorted(k for k in z.files if k.startswith(name + "_") and k.split("_", 1)[1][0] in "pst0123456789")
-    assert files
-    for key in files:
-        stem = key.split("_", 1)[1]
-        got = np.load(wd / f"{stem}.npy")
-        assert _same(got, z[key]), (key, np.nanmax(np.abs(got.astype(np.float64) - z[key])) if got.dtype.kind == "f"
-                                    else int((got != z[key]).sum()))
+    stems = [k.split("_", 1)[1] for k in z.files if k.startswith(name + "_")]
+    stems = [s for s in stems if s.isdigit() or (s[0] in "pst" and s[1:].isdigit())]
+    assert stems
+    report = []
+    for stem in sorted(stems, key=lambda s: (s.isdigit(), s)):       # pass-1 files first, then final maps
+        got, want = np.load(wd / f"{stem}.npy"), z[f"{name}_{stem}"]
+        if _same(got, want):
+            continue
+        g, w = got.astype(np.float64), want.astype(np.float64)
+        diff = (g != w) & ~(np.isnan(g) & np.isnan(w))
+        both = np.isfinite(g) & np.isfinite(w) & (w != 0)
+        rel = np.abs(g[both] - w[both]) / np.abs(w[both])
+        report.append((stem, int(diff.sum()), float(np.mean(rel <= 2e-3)) if rel.size else 1.0))
+        assert diff.mean() <= 0.005 and (rel.size == 0 or np.mean(rel <= 2e-3) >= 0.99), report
+    print("densify", name, "files differing from the reference:", report)
     if name == "pm":   # the repair pass changed pixels: final maps are not just the filtered pass-1 maps
         moved = sum(int((np.nan_to_num(z[f"pm_{k:05d}"]) != np.nan_to_num(z[f"pm_p{k:05d}"])).sum())
                     for k in range(9) if f"pm_{k:05d}" in z.files)
         assert moved > 100, moved
-    for field in ("xyz", "rgb", "views", "cam"):
-        assert _same(getattr(cloud, field), z[f"{name}_{field}"]), field
+    want = z[f"{name}_xyz"]
+    have = {row.tobytes() for row in np.ascontiguousarray(cloud.xyz)}
+    found = np.mean([row.tobytes() in have for row in np.ascontiguousarray(want)])
+    print("densify", name, "points", len(cloud.xyz), "reference", len(want), "found bit for bit", found)
+    assert abs(len(cloud.xyz) - len(want)) <= 0.01 * len(want) and found >= 0.98
     assert cloud.spacing == float(z[f"{name}_spacing"])
 
 
@@ -72,3 +88,22 @@ def test_densify_keeps_no_files_by_default(tmp_path):
     opt = mvs.DepthOptions(**SETTINGS["plain"])
     cloud = densify.densify(ar, rec, gains, biases, str(tmp_path), opt, load_rgb=load_rgb, load_mask=load_mask)
     assert len(cloud.xyz) == len(z["plain_xyz"]) and not (tmp_path / "depthmaps").exists()
+
+
+@pytest.mark.parametrize("name", list(SETTINGS))
+def test_densify_exact_matches_reference_bit_for_bit(name, tmp_path):
+    """exact=True: every sweep in the reference's summation order -- the cloud and every depth-map
+    file equal the reference's bit for bit."""
+    from py_ortho_amd import densify, mvs
+    z = np.load(GOLDEN)
+    ar, rec, gains, biases, load_rgb, load_mask = _inputs(z)
+    opt = mvs.DepthOptions(keep_depthmaps=True, workers=2, **SETTINGS[name])
+    cloud = densify.densify(ar, rec, gains, biases, str(tmp_path), opt, load_rgb=load_rgb, load_mask=load_mask,
+                            exact=True)
+    wd = tmp_path / "depthmaps"
+    stems = [k.split("_", 1)[1] for k in z.files if k.startswith(name + "_")]
+    stems = [s for s in stems if s.isdigit() or (s[0] in "pst" and s[1:].isdigit())]
+    for stem in stems:
+        assert _same(np.load(wd / f"{stem}.npy"), z[f"{name}_{stem}"]), stem
+    for field in ("xyz", "rgb", "views", "cam"):
+        assert _same(getattr(cloud, field), z[f"{name}_{field}"]), field

@@ -316,3 +316,24 @@ def test_np_init_normals_matches_reference_golden():
     for k in z["views"]:
         got = cases.np_init_normals(z[f"depth_{k}"], z[f"ray_{k}"])
         assert _same(got, z[f"normals_{k}"])
+
+
+def test_combine_many_sources_sums_top_k_in_descending_order():
+    """orc_combine with more than 16 sources: the mean of the top k scores summed largest first in
+    float32 (the loop of _dense.pyx:124-133 for any n; the reference's own buffer holds 16)."""
+    rng = np.random.default_rng(8)
+    n, H, W, D = 23, 5, 6, 7
+    scores = rng.uniform(-1, 1, (D, n, H, W)).astype(np.float32)
+    for k in (1, 6, 23):
+        st, idx = oracle.new_state(H, W)
+        means = np.empty((D, H, W), np.float32)
+        for i in range(D):
+            oracle.combine(scores[i], k, i, *st, idx)
+            srt = -np.sort(-scores[i], axis=0)
+            acc = np.zeros((H, W), np.float32)
+            for a in range(k):
+                acc = (acc + srt[a]).astype(np.float32)
+            means[i] = (acc / np.float32(k)).astype(np.float32)
+            assert _same(st[1], means[i])                       # prev = this hypothesis's mean
+        assert np.array_equal(idx, np.argmax(means, axis=0))
+        assert _same(st[0], means.max(0))
