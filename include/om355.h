@@ -245,8 +245,8 @@ int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream);
 /* The precision variant of om_sweep (no reference counterpart: the reference computes in FP64 on
  * the CPU, _dense.pyx:740-838): reference (ref16) and source images (src[s].img) stored in FP16 or
  * BF16, warp and bilinear in FP32, window sums and NCC in FP32, plane depth and the variance guard in
- * FP64.  Hypotheses [i0, i0 + nh) continue the desc's winner state (one chunk); distortion-free
- * sources; the state is finished by om_sweep_finalize.  Errors against the FP64 path are stated in
+ * FP64.  Hypotheses [i0, i0 + nh) continue the desc's winner state (one chunk); the radial model
+ * (k1..k3) is evaluated in FP32; the state is finished by om_sweep_finalize.  Errors against the FP64 path are stated in
  * the tests and benchmark. */
 int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref16, void *stream);
 
@@ -259,7 +259,7 @@ int om_pm_workspace_counts(const void *workspace, int32_t H, int32_t W, int64_t 
  * _dense.pyx:461-556): source images imgs16[s] stored as FP16 or BF16 (fmt), the plane (num = d n.ray)
  * and the geometric term in FP64, per sample the ray-plane intersection, projection and bilinear tap
  * in FP32, window sums in FP32, the variances and their guard in FP64.  Same red-black order and random
- * stream as om_pm_refine; distortion-free sources; same workspace. */
+ * stream as om_pm_refine; the radial model (k1..k3) in FP32; same workspace. */
 int om_pm_refine_lp(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
                     const void *const *imgs16, const float *const *deps, const int32_t *ih, const int32_t *iw,
                     const double *par, float *depth, float *normal, float *score, const uint8_t *active, int32_t r,

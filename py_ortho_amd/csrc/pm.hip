@@ -128,8 +128,9 @@ struct PMCam {
     int iw, ih;
     int pad[2];
     double wl, hl;  // iw - 1.001, ih - 1.001: the bilinear's bounds (_dense.pyx:451)
-    // precision variant only: R (9) and Cr - C (3) rounded to float, f, cx, cy, bounds
+    // precision variant only: R (9) and Cr - C (3) rounded to float, f, cx, cy, bounds, k1..k3
     float rf[9], bf[3], ff, cxf, cyf, wlf, hlf;
+    float k1f, k2f, k3f;
     int pad2;
 };
 
@@ -317,7 +318,13 @@ __device__ __forceinline__ void pm_ncc_lp(const PMK &P, int y, int x, double num
                         const float xc = fmaf(cs.rf[0], dx, fmaf(cs.rf[1], dy, cs.rf[2] * dz));
                         const float yc = fmaf(cs.rf[3], dx, fmaf(cs.rf[4], dy, cs.rf[5] * dz));
                         const float iz = __builtin_amdgcn_rcpf(zc);
-                        const float pu = fmaf(cs.ff, xc * iz, cs.cxf), pv = fmaf(cs.ff, yc * iz, cs.cyf);
+                        const float nx = xc * iz, ny = yc * iz;
+                        float dd = cs.ff;   // radial model of _dense.pyx:438-442 in FP32
+                        if (!P.pinhole) {
+                            const float r2 = fmaf(nx, nx, ny * ny);
+                            dd = cs.ff * fmaf(r2, fmaf(r2, fmaf(r2, cs.k3f, cs.k2f), cs.k1f), 1.0f);
+                        }
+                        const float pu = fmaf(dd, nx, cs.cxf), pv = fmaf(dd, ny, cs.cyf);
                         if (pu >= 0.f && pv >= 0.f && pu <= cs.wlf && pv <= cs.hlf) {
                             const int x0 = (int)pu, y0 = (int)pv;
                             const float fx = pu - (float)x0, fy = pv - (float)y0;
@@ -1452,9 +1459,10 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
         c.cyf = (float)c.par[17];
         c.wlf = (float)c.wl;
         c.hlf = (float)c.hl;
+        c.k1f = (float)c.par[13];
+        c.k2f = (float)c.par[14];
+        c.k3f = (float)c.par[15];
     }
-    if (fmt != 0 && !P.pinhole)
-        return fail(OM_EUNSUP, "the precision variant of pm_refine supports distortion-free sources only");
     hipStream_t st = as_stream(stream);
     int rc;
     using RunFn = int (*)(PMK &, const PMCamArgs &, int, hipStream_t, uint32_t, const PMWs &);

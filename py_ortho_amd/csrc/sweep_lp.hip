@@ -36,6 +36,7 @@ constexpr int kMaxR = 7;
 struct SrcLP {
     const uint16_t *img;
     int h, w;
+    float k1, k2, k3;  // radial distortion (0 for a pinhole)
     float g[12];  // Rs (9, row-major) and b = Rs (C_ref - C_src) (3), rounded from FP64
     float f, cx, cy, wl, hl;  // focal, principal point, w - 1.001, h - 1.001
 };
@@ -46,6 +47,7 @@ struct SweepLPK {
     int H, W, n, r, k, mode;
     const float *ha, *hb;
     int hoff, i0, nh, tiles_x, tiles_y;
+    int pinhole;              // every source has k1 = k2 = k3 = 0
     float *best, *prev, *spb, *snb;
     int *idx;
     SrcLP src[16];
@@ -196,7 +198,13 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                     const float Z = fmaf(d, fmaf(g[6], rx, fmaf(g[7], ry, g[8] * rz)), g[11]);
                     if (Z > 1e-6f) {
                         const float iz = __builtin_amdgcn_rcpf(Z);
-                        const float u = fmaf(S.f, X * iz, S.cx), v = fmaf(S.f, Y * iz, S.cy);
+                        const float nx = X * iz, ny = Y * iz;
+                        float dd = S.f;   // radial model of _sweep.h:55-56 in FP32
+                        if (!P.pinhole) {
+                            const float r2 = fmaf(nx, nx, ny * ny);
+                            dd = S.f * fmaf(r2, fmaf(r2, fmaf(r2, S.k3, S.k2), S.k1), 1.0f);
+                        }
+                        const float u = fmaf(dd, nx, S.cx), v = fmaf(dd, ny, S.cy);
                         if (u >= 0.f && v >= 0.f && u <= S.wl && v <= S.hl) {
                             const int x0 = (int)u, y0 = (int)v;
                             const float fx = u - (float)x0, fy = v - (float)y0;
@@ -373,6 +381,7 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
         return fail(OM_EINVAL, "unknown hypothesis generator %d", d->hyp_mode);
     SweepLPK P;
     memset(&P, 0, sizeof P);
+    P.pinhole = 1;
     P.ref = (const uint16_t *)ref16;
     P.mu = d->mu;
     P.sd = d->sd;
@@ -400,9 +409,12 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
         if (!tap_size_ok(S.h, S.w))
             return fail(OM_ESHAPE, "source %d: images of 2^30 pixels or more, or with a side of 2^24 or more, are "
                                    "not supported", s);
-        if (S.geo[13] != 0.0 || S.geo[14] != 0.0 || S.geo[15] != 0.0)
-            return fail(OM_EUNSUP, "reduced-precision sweep supports distortion-free sources only");
         SrcLP &Q = P.src[s];
+        Q.k1 = (float)S.geo[13];
+        Q.k2 = (float)S.geo[14];
+        Q.k3 = (float)S.geo[15];
+        if (S.geo[13] != 0.0 || S.geo[14] != 0.0 || S.geo[15] != 0.0)
+            P.pinhole = 0;
         Q.img = (const uint16_t *)S.img;
         Q.h = S.h;
         Q.w = S.w;

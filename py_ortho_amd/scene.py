@@ -108,10 +108,18 @@ def texture(terr: Terrain, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
 
 
 def camera_rays(cam: Camera, device, dtype=torch.float64) -> torch.Tensor:
-    """(3, h, w) world directions with camera-axis depth 1 (pinhole; distortion-free cameras)."""
+    """(3, h, w) world directions with camera-axis depth 1; a radially distorted camera's pixels
+    are undistorted by 20 fixed-point steps of x = n / (1 + r2 (k1 + r2 (k2 + r2 k3)))."""
     v, u = torch.meshgrid(torch.arange(cam.h, device=device, dtype=dtype),
                           torch.arange(cam.w, device=device, dtype=dtype), indexing="ij")
-    xc = torch.stack([(u - cam.cx) / cam.f, (v - cam.cy) / cam.f, torch.ones_like(u)])
+    nx, ny = (u - cam.cx) / cam.f, (v - cam.cy) / cam.f
+    x, y = nx, ny
+    if cam.k1 or cam.k2 or cam.k3:
+        for _ in range(20):
+            r2 = x * x + y * y
+            d = 1 + r2 * (cam.k1 + r2 * (cam.k2 + r2 * cam.k3))
+            x, y = nx / d, ny / d
+    xc = torch.stack([x, y, torch.ones_like(u)])
     R = torch.as_tensor(cam.R, dtype=dtype, device=device)
     return torch.einsum("ji,jhw->ihw", R, xc)
 
@@ -178,8 +186,9 @@ class Scene:
 
 
 def make_scene(seed: int, W: int, H: int, f: float, alt: float, src_xy: Sequence[Tuple[float, float]],
-               device="cpu", noise: float = 0.01, ref_xy=(0.0, 0.0)) -> Scene:
-    """Reference camera at (ref_xy, alt) and one source camera per (x, y), all nadir."""
+               device="cpu", noise: float = 0.01, ref_xy=(0.0, 0.0), k=(0.0, 0.0, 0.0)) -> Scene:
+    """Reference camera at (ref_xy, alt) and one source camera per (x, y), all nadir, all with the
+    radial distortion k = (k1, k2, k3)."""
     half_w = (W / 2) / f * alt
     half_h = (H / 2) / f * alt
     xs = [p[0] for p in src_xy] + [ref_xy[0]]
@@ -187,6 +196,8 @@ def make_scene(seed: int, W: int, H: int, f: float, alt: float, src_xy: Sequence
     ext = (min(xs) - half_w, max(xs) + half_w, min(ys) - half_h, max(ys) + half_h)
     terr = make_terrain(seed, ext)
     cams = [pinhole((ref_xy[0], ref_xy[1], alt), f, W, H)] + [pinhole((x, y, alt), f, W, H) for x, y in src_xy]
+    for c in cams:
+        c.k1, c.k2, c.k3 = (float(v) for v in k)
     views = []
     for k, cam in enumerate(cams):
         img, dep = render(terr, cam, noise_seed=seed * 1000 + 2 + k, noise=noise, device=device)
@@ -194,20 +205,22 @@ def make_scene(seed: int, W: int, H: int, f: float, alt: float, src_xy: Sequence
     return Scene(terr, views[0], views[1:])
 
 
-def config_c1(device="cpu", scale: float = 1.0) -> Scene:
-    """Plain plane sweep: 640x480, f=600, 80 m, 4 sources on a 15 m cross (seed 0)."""
+def config_c1(device="cpu", scale: float = 1.0, k=(0.0, 0.0, 0.0)) -> Scene:
+    """Plain plane sweep: 640x480, f=600, 80 m, 4 sources on a 15 m cross (seed 0); k: radial
+    distortion of every camera (0 in the plain configuration)."""
     W, H = int(round(640 * scale)), int(round(480 * scale))
     b = 15.0
-    return make_scene(0, W, H, 600.0 * scale, 80.0, [(b, 0.0), (-b, 0.0), (0.0, b), (0.0, -b)], device=device)
+    return make_scene(0, W, H, 600.0 * scale, 80.0, [(b, 0.0), (-b, 0.0), (0.0, b), (0.0, -b)], device=device, k=k)
 
 
 C2_SOURCES = [(x, y) for y in (-10.0, 10.0) for x in (-30.0, -10.0, 10.0, 30.0)]
 
 
-def config_c2(device="cpu", scale: float = 1.0) -> Scene:
-    """Single-view PatchMatch: 2000x1500, f=1800, 100 m, 8 sources on a 2x4 grid, 20 m baseline."""
+def config_c2(device="cpu", scale: float = 1.0, k=(0.0, 0.0, 0.0)) -> Scene:
+    """Single-view PatchMatch: 2000x1500, f=1800, 100 m, 8 sources on a 2x4 grid, 20 m baseline;
+    k: radial distortion of every camera (0 in the configuration)."""
     W, H = int(round(2000 * scale)), int(round(1500 * scale))
-    return make_scene(3, W, H, 1800.0 * scale, 100.0, C2_SOURCES, device=device)
+    return make_scene(3, W, H, 1800.0 * scale, 100.0, C2_SOURCES, device=device, k=k)
 
 
 def config_c2_cameras(scale: float = 1.0) -> Scene:

@@ -58,14 +58,38 @@ def test_sweep_lp_error_against_fp64(fmt, r):
     assert stats[2] < tol[0] and stats[3] < tol[1] and stats[4] > tol[2] and stats[5] < tol[3], stats
 
 
-def test_sweep_lp_rejects_distortion():
+K_DISTORT = (-0.06, 0.015, -0.002)
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_sweep_lp_distorted_cameras_against_fp64(fmt):
+    """Radially distorted cameras (k1, k2, k3 = -0.06, 0.015, -0.002 on every view, images rendered
+    through the distortion): the FP16 / BF16 sweep evaluates the radial model in FP32; same stated
+    bounds as the pinhole scene at r = 2."""
     from py_ortho_amd import dense, scene
-    sc = _scene()
+    sc = scene.config_c1(device="cuda", scale=0.75, k=K_DISTORT)
     ref = sc.ref.img.contiguous()
-    mu, sd = dense.ref_stats(ref, 2)
-    geo = sc.geo().copy()
-    geo[0, 13] = 0.01
-    hyp = {"mode": "const", "inv": torch.full((4,), 1 / 70.0, device="cuda")}
-    with pytest.raises(NotImplementedError):
-        dense.sweep_lp(dense.to_lp(ref, "f16"), mu, sd, [dense.to_lp(v.img.contiguous(), "f16") for v in sc.srcs],
-                       scene.rays_world(sc.ref.cam, "cuda"), geo, hyp, 2, 2, "f16")
+    c = sc.ref.cam
+    H, W = ref.shape
+    ray = dense.rays_world(H, W, c.R, c.cx, c.cy, c.f, c.k1, c.k2, c.k3)
+    r = 2
+    mu, sd = dense.ref_stats(ref, r)
+    srcs = [v.img.contiguous() for v in sc.srcs]
+    geo = sc.geo()
+    assert (geo[:, 13] != 0).all()
+    planes = torch.linspace(60.0, 85.0, 96, dtype=torch.float64)
+    hyp = {"mode": "const", "inv": (1.0 / planes).to(torch.float32).cuda()}
+    st64 = dense.sweep(ref, mu, sd, srcs, ray, geo, hyp, r, 2)
+    d64 = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, r, 2, st64)
+    stlp = dense.sweep_lp(dense.to_lp(ref, fmt), mu, sd, [dense.to_lp(s, fmt) for s in srcs], ray, geo, hyp, r, 2,
+                          fmt)
+    dlp = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, r, 2, stlp)
+    valid = (st64.best > 0.3) & (64.0 * sd > 0.5)
+    assert valid.float().mean() > 0.5                      # the distorted scene matches
+    db = (stlp.best - st64.best).abs()[valid]
+    same = (stlp.idx == st64.idx)[valid].float().mean().item()
+    rel = ((dlp - d64).abs() / d64)[valid]
+    stats = (fmt, float(db.median()), float(torch.quantile(db, 0.99)), same, float(torch.quantile(rel, 0.99)))
+    print("lp error, distorted", stats)
+    tol = {"f16": (1e-2, 0.1, 0.98, 1e-3), "bf16": (0.06, 0.5, 0.85, 2e-2)}[fmt]
+    assert stats[1] < tol[0] and stats[2] < tol[1] and stats[3] > tol[2] and stats[4] < tol[3], stats
