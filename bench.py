@@ -28,7 +28,9 @@ import ctypes
 import json
 import math
 import os
+import queue
 import sys
+import threading
 import time
 
 import numpy as np
@@ -242,8 +244,8 @@ def cpu_baseline_pm(c, torch, args):
     # call's fixed cost, which made a single small probe pick too small a crop)
     t1, t2 = crop(24, 1, 1), crop(48, 1, 1)
     per_px_it = (t2 - t1) / (48 * 48 - 24 * 24) if t2 > t1 else t2 / (48 * 48)
-    target = 1.5                                   # seconds of work per core (bounded sample)
-    side = int(max(16, min(256, math.sqrt(target / (per_px_it * c["iters"])))))
+    target = 20.0                                  # seconds of work per core (bounded sample)
+    side = int(max(128, min(256, math.sqrt(target / (per_px_it * c["iters"])))))
     wall = crop(side, c["iters"], cores)
     evals = cores * side * side * 11 * c["iters"]
     return {"value": evals / wall / 1e6, "unit": "Mpixel-hyp/s", "cores": cores, "kind": "port",
@@ -575,7 +577,6 @@ def c5_requests(pools, rate, duration, seed=6):
 
 
 def run_c5(args, torch):
-    import threading
     from py_ortho_amd import block, dense, serve
     t_scene = time.perf_counter()
     pools = _pools_c5(torch, args.scene_cache, args.scale)
@@ -595,7 +596,43 @@ def run_c5(args, torch):
     for f in [srv.submit(r) for _, r in warm]:
         f.result()
     arrivals = c5_requests(pools, args.rate, args.duration)
-    futs = [None] * len(arrivals)
+    N = len(arrivals)
+    n_plain = min(N, args.plain_requests)
+    # results are summarised as they complete (a 120 s stream is thousands of depth maps): the latency
+    # of every request, the accuracy of a sample of them, the full maps of the first n_plain only
+    lat = [0.0] * N
+    t_done = [0.0] * N
+    kept = {}
+    acc_every = max(1, N // 200)
+    acc_q: "queue.Queue" = queue.Queue()
+    rel_parts = []
+    done = threading.Event()
+    lock = threading.Lock()
+    n_done = [0]
+
+    def on_done(j, fut):
+        r = fut.result()
+        lat[j] = r.latency * 1e3
+        t_done[j] = r.t_done
+        if j < n_plain:
+            kept[j] = r
+        if j % acc_every == 0:
+            acc_q.put((j, r.depth))
+        with lock:
+            n_done[0] += 1
+            if n_done[0] == N:
+                done.set()
+
+    def accuracy_worker():
+        while True:
+            item = acc_q.get()
+            if item is None:
+                return
+            j, dep = item
+            t = truth[arrivals[j][1].ref].cpu().numpy()
+            e = np.abs(dep - t) / t
+            rel_parts.append(e[np.isfinite(e)])
+
     t0 = time.perf_counter()
 
     def feed():
@@ -603,28 +640,34 @@ def run_c5(args, torch):
             dt = t0 + ta - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            futs[j] = srv.submit(req, t_arrival=t0 + ta)
+            f = srv.submit(req, t_arrival=t0 + ta)
+            f.add_done_callback(lambda fut, j=j: on_done(j, fut))
 
     from py_ortho_amd import _lib
     lib = _lib.lib()
     lib.om_profile_enable(1)
     prof_read(lib, "sweep_multi")
+    aw = threading.Thread(target=accuracy_worker, daemon=True)
+    aw.start()
+    progress(f"c5: {N} requests arriving over {args.duration:g} s")
     th = threading.Thread(target=feed)
     th.start()
+    while not done.wait(30.0):
+        progress(f"c5: {n_done[0]}/{N} requests finished after {time.perf_counter() - t0:.0f} s")
     th.join()
-    res = [f.result() for f in futs]
     srv.stop()
+    acc_q.put(None)
+    aw.join()
     lib.om_profile_enable(0)
     step_ms, step_n = prof_read(lib, "sweep_multi")
     lib.om_profile_read(None, None, None)
-    t_end = max(r.t_done for r in res)
+    t_end = max(t_done)
     px_h = [images[q.ref][0].size * q.planes for _, q in arrivals]
     flops = sum(images[q.ref][0].size * q.planes * sweep_flops_per_hyp(len(q.srcs)) for _, q in arrivals)
     wall = t_end - t0
-    lat = [r.latency * 1e3 for r in res]
     served = sum(px_h) / wall / 1e6
+    progress("c5: plain per-request comparison")
     # plain per-request runs (one request at a time: fused single-view sweep + parabola), same inputs
-    n_plain = min(len(arrivals), args.plain_requests)
     torch.cuda.synchronize()
     tp = time.perf_counter()
     same = True
@@ -643,20 +686,15 @@ def run_c5(args, torch):
         d = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, window, top_k, st).cpu().numpy()
         for k in q.srcs:
             store.unpin(k)
-        same = same and np.array_equal(d.view(np.uint32), res[j].depth.view(np.uint32))
+        same = same and np.array_equal(d.view(np.uint32), kept[j].depth.view(np.uint32))
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - tp
     plain = sum(px_h[:n_plain]) / plain_s / 1e6
-    rel = []
-    for (ta, q), r in zip(arrivals, res):
-        t = truth[q.ref].cpu().numpy()
-        e = np.abs(r.depth - t) / t
-        rel.append(e[np.isfinite(e)])
-    rel = np.concatenate(rel)
+    rel = np.concatenate(rel_parts)
     classes = {}
-    for (ta, q), r in zip(arrivals, res):
+    for j, (ta, q) in enumerate(arrivals):
         H, W = images[q.ref][0].shape
-        classes.setdefault(f"{W}x{H}", []).append(r.latency * 1e3)
+        classes.setdefault(f"{W}x{H}", []).append(lat[j])
     res_line = {
         "metric": METRIC, "value": served, "unit": "Mpixel-hyp/s", "n_gpus": 1, "steps": len(arrivals),
         "warmup": len(warm), "ms_per_step": wall * 1e3 / max(len(arrivals), 1), "higher_is_better": True,
@@ -674,7 +712,8 @@ def run_c5(args, torch):
                        "by_class_p99": {k: _percentile(v, 99) for k, v in classes.items()}},
         "plain_per_request": {"requests": n_plain, "mpix_hyp_s": plain, "speedup": served / plain,
                               "served_equals_plain": bool(same)},
-        "accuracy": {"frac_within_1pct": float((rel < 0.01).mean()), "median_rel_err": float(np.median(rel))},
+        "accuracy": {"frac_within_1pct": float((rel < 0.01).mean()), "median_rel_err": float(np.median(rel)),
+                     "requests_sampled": len(rel_parts)},
         "roofline": {"bound": "valu", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
                      "achieved": flops / (step_ms / 1e3) / 1e12 if step_ms else 0.0, "peak": FP64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": flops / (step_ms / 1e3) / FP64_PEAK if step_ms else 0.0,
@@ -705,8 +744,8 @@ def cpu_baseline_sweep(images, arrivals, window, top_k, args):
 
 
 def cpu_sweep_rate(ref, mu, sd, ray, srcs, geo, planes, window, top_k, args, what):
-    """The oracle's sweep (FP64, the reference's arithmetic) over all planes on a centre crop sized
-    to ~1.5 s per core, `cores` concurrent copies (one view per thread, densify.py:596):
+    """The oracle's sweep (FP64, the reference's arithmetic) over all planes on a centre crop of
+    256x256 to 512x512 (~12 s per core), `cores` concurrent copies (one view per thread, densify.py:596):
     Mpixel-hyp/s = cores * crop pixels * planes / wall."""
     import oracle
     cores = _cores() if args.cpu_threads <= 0 else args.cpu_threads
@@ -725,7 +764,7 @@ def cpu_sweep_rate(ref, mu, sd, ray, srcs, geo, planes, window, top_k, args, wha
         return time.perf_counter() - t
 
     per = sweep(16, 8, 1) / (256 * 8)
-    side = int(max(16, min(min(H, W), 512, math.sqrt(1.5 / (per * nplanes)))))
+    side = int(max(min(256, H, W), min(min(H, W), 512, math.sqrt(12.0 / (per * nplanes)))))
     wall = sweep(side, nplanes, cores)
     return {"value": cores * side * side * nplanes / wall / 1e6, "unit": "Mpixel-hyp/s", "cores": cores,
             "kind": "port", "sample": f"{cores} concurrent copies of the oracle sweep ({nplanes} planes, "
