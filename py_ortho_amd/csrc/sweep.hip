@@ -115,32 +115,45 @@ constexpr int kTWP = kTW + 1;
 
 __host__ __device__ constexpr int halo_stride(int r) { return (kTW + 2 * r) | 1; }
 
-// LDS bytes of one workgroup: per halo pixel the pixel index, reference and warped source (and the
-// depth of the current plane unless the planes are constant over the image), plus four planes of
-// horizontal sums.
+// Constant planes are warped in groups of kPlaneGroup per pass over the halo: the plane-independent
+// part of the projection, Rs * ray (_sweep.h:51-53 forms it as its own parenthesised term), is
+// computed once per halo pixel and source for the group.
+#ifndef OM_SWEEP_GROUP
+#define OM_SWEEP_GROUP 2
+#endif
+constexpr int kPlaneGroup = OM_SWEEP_GROUP;
+
+__host__ __device__ constexpr int plane_group(bool constd) { return constd ? kPlaneGroup : 1; }
+
+// LDS bytes of one workgroup.  Constant planes: per halo pixel the reference and one warped source
+// value per plane of the group (the pixel index is recomputed from the halo position).  Otherwise:
+// per halo pixel the depth of the current plane, the pixel index, the reference and the warped
+// source.  Both: four planes of horizontal sums.
 __host__ __device__ constexpr size_t sweep_lds(int th, int r, bool constd)
 {
-    return (size_t)(th + 2 * r) * halo_stride(r) * (constd ? 12 : 20) + (size_t)4 * (th + 2 * r) * kTWP * 4;
+    return (size_t)(th + 2 * r) * halo_stride(r) * (constd ? 4 + 4 * plane_group(true) : 20) +
+           (size_t)4 * (th + 2 * r) * kTWP * 4;
 }
 
 // One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
 // device memory (batched launches).  CD: the hypotheses are constant planes (OM_HYP_CONST), so the
-// depth of a plane is one value for the whole tile and needs no LDS.  HALF: the horizontal pass's
-// half-wave task order (below).
+// depth of a plane is one value for the whole tile and needs no LDS; they are warped in groups of
+// G = kPlaneGroup.  HALF: the horizontal pass's half-wave task order (below).
 template <int K, int VR, bool CD, bool HALF, class D>
 __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, double *lds_d)
 {
+    constexpr int G = plane_group(CD);
     const int r = P.r;
     constexpr int TH = kVSeg * VR;
     const int HR = TH + 2 * r;          // halo rows
     const int HC = kTW + 2 * r;         // halo columns
     const int HCP = halo_stride(r);     // padded row stride
-    const int NHP = HR * HCP;           // padded halo; the pad column holds index -1
-    double *sD = lds_d;                                   // (HR, HCP) depth of the current hypothesis (!CD)
-    int *sP = (int *)(CD ? lds_d : (void *)(sD + NHP));   // (HR, HCP) pixel index, -1 outside the image
-    float *sI = (float *)(sP + NHP);                      // (HR, HCP) reference
-    float *sJ = sI + NHP;                                 // (HR, HCP) warped source, NaN = invalid
-    float *sH = sJ + NHP;                                 // (4, HR, kTWP) horizontal sums
+    const int NHP = HR * HCP;           // padded halo; the pad column is outside the image
+    double *sD = lds_d;                                          // (HR, HCP) depth of the current hypothesis (!CD)
+    int *sP = CD ? nullptr : (int *)(sD + NHP);                  // (HR, HCP) pixel index, -1 outside (!CD)
+    float *sI = CD ? (float *)lds_d : (float *)(sP + NHP);       // (HR, HCP) reference
+    float *sJ = sI + NHP;                                        // (G, HR, HCP) warped source, NaN = invalid
+    float *sH = sJ + G * NHP;                                    // (4, HR, kTWP) horizontal sums
 
     const int tid = threadIdx.x;
     const int tx0 = (tile % P.tiles_x) * kTW;
@@ -150,15 +163,20 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const int H = P.H, W = P.W;
     const size_t HW = (size_t)H * W;
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
-
-    // ---- the tile + halo: pixel index and reference value
-    for (int hp = tid; hp < NHP; hp += kThreads) {
+    // image pixel of halo pixel hp, -1 outside the image (or in the pad column)
+    auto halo_px = [&](int hp) -> int {
         const int hy = hp / HCP, hx = hp - hy * HCP;
         const int gy = ty0 - r + hy, gx = tx0 - r + hx;
-        const bool in = hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W;
-        const int p = in ? gy * W + gx : -1;
-        sP[hp] = p;
-        sI[hp] = in ? P.ref[p] : 0.f;
+        return (hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W) ? gy * W + gx : -1;
+    };
+    auto pix = [&](int hp) -> int { return CD ? halo_px(hp) : sP[hp]; };
+
+    // ---- the tile + halo: pixel index (unless recomputed) and reference value
+    for (int hp = tid; hp < NHP; hp += kThreads) {
+        const int p = halo_px(hp);
+        if (!CD)
+            sP[hp] = p;
+        sI[hp] = p >= 0 ? P.ref[p] : 0.f;
     }
 
     // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+VR-1
@@ -167,7 +185,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     float mu_[VR], sd_[VR];
     float st_best[VR], st_prev[VR], st_spb[VR], st_snb[VR], st_first[VR];
     int st_idx[VR];
-    float tk[VR][K];
+    float tk[G][VR][K];
     const bool chunked = P.chunks > 1;
 #pragma unroll
     for (int j = 0; j < VR; ++j) {
@@ -194,13 +212,20 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const double kk = 1.0 / (double)((2 * r + 1) * (2 * r + 1));
     __syncthreads();
 
-    for (int i = ia; i < ib; ++i) {
-        // inverse depth -> depth, per halo pixel (shared by all sources) or once for a constant
+    for (int i = ia; i < ib; i += G) {
+        const int ng = min(G, ib - i);   // planes of this group (wave-uniform)
+        // inverse depth -> depth, per halo pixel (shared by all sources) or once per constant
         // plane; the previous hypothesis's last warp pass finished reading sD before the barrier
         // after it
-        double dc = 0.0;
+        double dc[G];
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg)
+            dc[gg] = 0.0;
         if (CD) {
-            dc = 1.0 / (double)P.ha[i - P.i0];
+#pragma unroll
+            for (int gg = 0; gg < G; ++gg)
+                if (gg < ng)
+                    dc[gg] = 1.0 / (double)P.ha[i + gg - P.i0];
         } else {
             for (int hp = tid; hp < NHP; hp += kThreads) {
                 const int p = sP[hp];
@@ -213,10 +238,12 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             }
         }
 #pragma unroll
-        for (int j = 0; j < VR; ++j)
+        for (int gg = 0; gg < G; ++gg)
 #pragma unroll
-            for (int a = 0; a < K; ++a)
-                tk[j][a] = -INFINITY;
+            for (int j = 0; j < VR; ++j)
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+                    tk[gg][j][a] = -INFINITY;
         if (!CD)
             __syncthreads();
 
@@ -226,120 +253,141 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             const double wl = (double)S.w - 1.001, hl = (double)S.h - 1.001;
             const float *img = S.img;
             const int sw = S.w;
-            // ---- W: warp the halo into LDS (_sweep.h:49-80).  Each step loads the ray of its next
-            // halo pixel (pixel 0's outside the image) before it works on its own.
+            // ---- W: warp the halo into LDS (_sweep.h:49-80) for every plane of the group.  Each
+            // step loads the ray of its next halo pixel (pixel 0's outside the image) before it
+            // works on its own.
             const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
                        *rayz = (const char *)(P.ray + 2 * HW);
-            int pn = tid < NHP ? sP[tid] : -1;
+            int pn = tid < NHP ? pix(tid) : -1;
             uint32_t on = (uint32_t)max(pn, 0) << 2;
             float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
                   rzn = *(const float *)(rayz + on);
             for (int hp = tid; hp < NHP; hp += kThreads) {
-                float J = NAN;
+                float J[G];
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg)
+                    J[gg] = NAN;
                 const int p = pn;
                 const double rx = rxn, ry = ryn, rz = rzn;
-                pn = hp + kThreads < NHP ? sP[hp + kThreads] : -1;
+                pn = hp + kThreads < NHP ? pix(hp + kThreads) : -1;
                 on = (uint32_t)max(pn, 0) << 2;
                 rxn = *(const float *)(rayx + on);
                 ryn = *(const float *)(rayy + on);
                 rzn = *(const float *)(rayz + on);
                 if (p >= 0) {
-                    const double d = CD ? dc : sD[hp];
-                    const double X = g[9] + d * ((g[0] * rx + g[1] * ry) + g[2] * rz);
-                    const double Y = g[10] + d * ((g[3] * rx + g[4] * ry) + g[5] * rz);
-                    const double Z = g[11] + d * ((g[6] * rx + g[7] * ry) + g[8] * rz);
-                    if (!(Z <= 1e-6)) {
-                        double nx, ny, u, v;
-                        div2_shared(X, Y, Z, nx, ny);
-                        if (P.pinhole) {
-                            u = g[12] * nx + g[16];  // f * (1 + r2 * 0) == f for finite r2
-                            v = g[12] * ny + g[17];
-                        } else {
-                            const double rr = nx * nx + ny * ny;
-                            const double dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
-                            u = dd * nx + g[16];
-                            v = dd * ny + g[17];
-                        }
-                        if (u >= 0.0 && v >= 0.0 && u <= wl && v <= hl) {
-                            const double val = bil_f64(img, sw, u, v);
-                            if (isfinite(val))
-                                J = (float)val;
+                    // Rs * ray, the plane-independent term of b + d * (Rs * ray)
+                    const double ax = (g[0] * rx + g[1] * ry) + g[2] * rz;
+                    const double ay = (g[3] * rx + g[4] * ry) + g[5] * rz;
+                    const double az = (g[6] * rx + g[7] * ry) + g[8] * rz;
+#pragma unroll
+                    for (int gg = 0; gg < G; ++gg) {
+                        if (gg < ng) {
+                            const double d = CD ? dc[gg] : sD[hp];
+                            const double X = g[9] + d * ax;
+                            const double Y = g[10] + d * ay;
+                            const double Z = g[11] + d * az;
+                            if (!(Z <= 1e-6)) {
+                                double nx, ny, u, v;
+                                div2_shared(X, Y, Z, nx, ny);
+                                if (P.pinhole) {
+                                    u = g[12] * nx + g[16];  // f * (1 + r2 * 0) == f for finite r2
+                                    v = g[12] * ny + g[17];
+                                } else {
+                                    const double rr = nx * nx + ny * ny;
+                                    const double dd = g[12] * (1.0 + rr * (g[13] + rr * (g[14] + rr * g[15])));
+                                    u = dd * nx + g[16];
+                                    v = dd * ny + g[17];
+                                }
+                                if (u >= 0.0 && v >= 0.0 && u <= wl && v <= hl) {
+                                    const double val = bil_f64(img, sw, u, v);
+                                    if (isfinite(val))
+                                        J[gg] = (float)val;
+                                }
+                            }
                         }
                     }
                 }
-                sJ[hp] = J;
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg)
+                    sJ[gg * NHP + hp] = J[gg];
             }
             __syncthreads();
-            // ---- H: horizontal window sums of one 8-column run per task (_sweep.h:85-114).  A
-            // wave takes 8 rows x 8 runs.  HALF: each half-wave takes 8 rows x 4 runs (lane l: row
-            // (l / 4) % 8, run l % 4 + 4 (l / 32)), and with the odd row strides every half-wave's
-            // addresses fall in distinct LDS banks (SQ_LDS_BANK_CONFLICT of a 2000x1500, S = 8,
-            // 128-plane sweep: 0 against 1.1e9 cycles; the sweep 0.4-1.7 % faster).  The batched
-            // kernel keeps 4 rows x 8 runs per half-wave: its register-bound build is 1.1 % slower
-            // with the half-wave order.
-            const int ntask = HALF ? (HR + 7) / 8 * 64 : HR * kRunsPerRow;
-            for (int t = tid; t < ntask; t += kThreads) {
-                int hr, c0;
-                if (HALF) {
-                    const int l = t & 63;
-                    hr = (t >> 6) * 8 + ((l >> 2) & 7);
-                    c0 = ((l & 3) + 4 * (l >> 5)) * kRun;
-                    if (hr >= HR)
-                        continue;
-                } else {
-                    hr = t / kRunsPerRow;
-                    c0 = (t % kRunsPerRow) * kRun;
-                }
-                const float *Jr = sJ + hr * HCP, *Ir = sI + hr * HCP;
-                int sv = 0;
-                double sj = 0.0, sjj = 0.0, sij = 0.0;
-                for (int t = 0; t <= 2 * r; ++t) {
-                    const float J = Jr[c0 + t];
-                    const bool ok = J == J;
-                    const float jv = ok ? J : 0.f;
-                    sv += ok;
-                    sj = sj + (double)jv;
-                    sjj = sjj + (double)(jv * jv);
-                    sij = sij + (double)(jv * Ir[c0 + t]);
-                }
-                float *h0 = sH + hr * kTWP, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
-                h0[c0] = (float)sv;
-                h1[c0] = (float)sj;
-                h2[c0] = (float)sjj;
-                h3[c0] = (float)sij;
 #pragma unroll
-                for (int c = c0 + 1; c < c0 + kRun; ++c) {
-                    const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
-                    const bool okn = Jn == Jn, oko = Jo == Jo;
-                    const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
-                    sv += (int)okn - (int)oko;
-                    sj = sj + (double)jn;
-                    sjj = sjj + (double)(jn * jn);
-                    sij = sij + (double)(jn * Ir[c + 2 * r]);
-                    sj = sj - (double)jo;
-                    sjj = sjj - (double)(jo * jo);
-                    sij = sij - (double)(jo * Ir[c - 1]);
-                    h0[c] = (float)sv;
-                    h1[c] = (float)sj;
-                    h2[c] = (float)sjj;
-                    h3[c] = (float)sij;
-                }
-            }
-            __syncthreads();
-            // ---- V: vertical sums and NCC (_sweep.h:116-178)
-            {
-                const float *h0 = sH + vc, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
-                double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-                for (int t = 0; t <= 2 * r; ++t) {
-                    const int o = (vy0 + t) * kTWP;
-                    c0 = c0 + (double)h0[o];
-                    c1 = c1 + (double)h1[o];
-                    c2 = c2 + (double)h2[o];
-                    c3 = c3 + (double)h3[o];
-                }
+            for (int gg = 0; gg < G; ++gg) {
+                if (gg >= ng)
+                    break;
+                if (gg > 0)
+                    __syncthreads();  // the previous plane's V pass has read sH
+                const float *sJg = sJ + gg * NHP;
+                // ---- H: horizontal window sums of one 8-column run per task (_sweep.h:85-114).
+                // A wave takes 8 rows x 8 runs.  HALF: each half-wave takes 8 rows x 4 runs (lane
+                // l: row (l / 4) % 8, run l % 4 + 4 (l / 32)), and with the odd row strides every
+                // half-wave's addresses fall in distinct LDS banks (SQ_LDS_BANK_CONFLICT of a
+                // 2000x1500, S = 8, 128-plane sweep: 0 against 1.1e9 cycles; the sweep 0.4-1.7 %
+                // faster).  The batched kernel keeps 4 rows x 8 runs per half-wave: its
+                // register-bound build is 1.1 % slower with the half-wave order.
+                const int ntask = HALF ? (HR + 7) / 8 * 64 : HR * kRunsPerRow;
+                for (int t = tid; t < ntask; t += kThreads) {
+                    int hr, c0;
+                    if (HALF) {
+                        const int l = t & 63;
+                        hr = (t >> 6) * 8 + ((l >> 2) & 7);
+                        c0 = ((l & 3) + 4 * (l >> 5)) * kRun;
+                        if (hr >= HR)
+                            continue;
+                    } else {
+                        hr = t / kRunsPerRow;
+                        c0 = (t % kRunsPerRow) * kRun;
+                    }
+                    const float *Jr = sJg + hr * HCP, *Ir = sI + hr * HCP;
+                    int sv = 0;
+                    double sj = 0.0, sjj = 0.0, sij = 0.0;
+                    for (int t = 0; t <= 2 * r; ++t) {
+                        const float J = Jr[c0 + t];
+                        const bool ok = J == J;
+                        const float jv = ok ? J : 0.f;
+                        sv += ok;
+                        sj = sj + (double)jv;
+                        sjj = sjj + (double)(jv * jv);
+                        sij = sij + (double)(jv * Ir[c0 + t]);
+                    }
+                    float *h0 = sH + hr * kTWP, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                    h0[c0] = (float)sv;
+                    h1[c0] = (float)sj;
+                    h2[c0] = (float)sjj;
+                    h3[c0] = (float)sij;
 #pragma unroll
-                for (int j = 0; j < VR; ++j) {
-                    {
+                    for (int c = c0 + 1; c < c0 + kRun; ++c) {
+                        const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
+                        const bool okn = Jn == Jn, oko = Jo == Jo;
+                        const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
+                        sv += (int)okn - (int)oko;
+                        sj = sj + (double)jn;
+                        sjj = sjj + (double)(jn * jn);
+                        sij = sij + (double)(jn * Ir[c + 2 * r]);
+                        sj = sj - (double)jo;
+                        sjj = sjj - (double)(jo * jo);
+                        sij = sij - (double)(jo * Ir[c - 1]);
+                        h0[c] = (float)sv;
+                        h1[c] = (float)sj;
+                        h2[c] = (float)sjj;
+                        h3[c] = (float)sij;
+                    }
+                }
+                __syncthreads();
+                // ---- V: vertical sums and NCC (_sweep.h:116-178)
+                {
+                    const float *h0 = sH + vc, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+                    for (int t = 0; t <= 2 * r; ++t) {
+                        const int o = (vy0 + t) * kTWP;
+                        c0 = c0 + (double)h0[o];
+                        c1 = c1 + (double)h1[o];
+                        c2 = c2 + (double)h2[o];
+                        c3 = c3 + (double)h3[o];
+                    }
+#pragma unroll
+                    for (int j = 0; j < VR; ++j) {
                         if (j > 0) {
                             const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
                             double t;
@@ -366,31 +414,35 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                             const double q = (mij - (double)mu_[j] * mj) / ((double)sd_[j] * sj);
                             out = q > 1.0 ? 1.0f : (q < -1.0 ? -1.0f : (float)q);
                         }
-                        topk_insert<K>(tk[j], out);
+                        topk_insert<K>(tk[gg][j], out);
                     }
                 }
             }
             // the next source's W pass writes sJ only (last read before the barrier above) and
             // its H pass writes sH after the next barrier, so no barrier is needed here
         }
-        // ---- combine + streaming winner (_sweep.h:181-214)
+        // ---- combine + streaming winner of each plane of the group, in order (_sweep.h:181-214)
 #pragma unroll
-        for (int j = 0; j < VR; ++j) {
-            {
+        for (int gg = 0; gg < G; ++gg) {
+            if (gg >= ng)
+                break;
+            const int ii = i + gg;
+#pragma unroll
+            for (int j = 0; j < VR; ++j) {
                 float s = 0.0f;
 #pragma unroll
                 for (int a = 0; a < K; ++a)
-                    s = s + tk[j][a];
+                    s = s + tk[gg][j][a];
                 s = s / (float)K;
-                if (i == ia)
+                if (ii == ia)
                     st_first[j] = s;
-                if (st_idx[j] == i - 1)
+                if (st_idx[j] == ii - 1)
                     st_snb[j] = s;
                 if (s > st_best[j]) {
                     st_best[j] = s;
                     st_spb[j] = st_prev[j];
                     st_snb[j] = chunked ? NAN : -2.0f;
-                    st_idx[j] = i;
+                    st_idx[j] = ii;
                 }
                 st_prev[j] = s;
             }
