@@ -187,6 +187,17 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 #define OM_PM_ATTR
 #endif
 
+// waves per SIMD of the random-refinement kernels (each lane a different pixel, two cost
+// evaluations with the pixel's state live around them)
+#ifndef OM_PM_RANDOM_WAVES
+#define OM_PM_RANDOM_WAVES OM_PM_WAVES
+#endif
+#if OM_PM_RANDOM_WAVES > 0
+#define OM_PM_RANDOM_ATTR __attribute__((amdgpu_waves_per_eu(OM_PM_RANDOM_WAVES, OM_PM_RANDOM_WAVES)))
+#else
+#define OM_PM_RANDOM_ATTR
+#endif
+
 #ifndef OM_PM_GROUP
 #define OM_PM_GROUP 4
 #endif
@@ -1112,7 +1123,7 @@ __global__ void __launch_bounds__(256) pm_rank_rows_kernel(const PMK P, const Pa
 __global__ void __launch_bounds__(256) pm_scan_rows_kernel(const PMK P, const PassK Q) { pm_scan_view(P, Q); }
 
 template <int SM, int FMT>
-__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_kernel(const PMK P, const PassK Q)
+__global__ void __launch_bounds__(kPMThreads) OM_PM_RANDOM_ATTR pm_random_kernel(const PMK P, const PassK Q)
 {
     const int n = pm_random_px<SM, FMT>(P, Q, (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     count_changes(P, block_total(n));
@@ -1152,7 +1163,7 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_propagate_batch(cons
 }
 
 template <int SM>
-__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_random_batch(const PMK *views, const int *blk0, int nviews,
+__global__ void __launch_bounds__(kPMThreads) OM_PM_RANDOM_ATTR pm_random_batch(const PMK *views, const int *blk0, int nviews,
                                                               const PassK Q)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
