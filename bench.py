@@ -41,6 +41,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpixel-hypotheses/s on 1 GPU; p50/p99 depth-map latency at fixed accuracy"
 FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
 HBM_PEAK = 8.0e12
+# HBM bytes of one full-block sweep_batch_kernel launch of the default workload (FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply; profiles/block_c3.md)
+C3_SWEEP_TRAFFIC = 3.37e11
 
 
 def progress(msg: str) -> None:
@@ -450,7 +453,8 @@ def run_c3(args, torch):
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
         "roofline": {"bound": "valu", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_batch_kernel",
+                     "frac": achieved / FP64_PEAK, "traffic": C3_SWEEP_TRAFFIC if args.scale == 1.0 else None,
+                     "kernel": "sweep_batch_kernel",
                      "kernel_avg_ms": sw_avg * 1e3, "launches": sw_n, "flops_per_step": flops,
                      "pm_propagate_ms": pr_ms, "pm_random_ms": rn_ms},
         "plain_single_view": plain, "batch_equals_single_view": same,
