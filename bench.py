@@ -915,9 +915,10 @@ def cpu_baseline_view(ref, mu, sd, ray, srcs, geo, par, Cr, deps, depth0, normal
                         np.ones((side, side), np.uint8), pr, pstep, k, 0.5, 0.01, it, dmin, dmax, 1)
         return time.perf_counter() - t
 
-    target = 1.5
-    side_s = int(max(16, min(512, math.sqrt(target / (sweep(16, 8, 1) / (256 * 8) * D)))))
-    side_p = int(max(16, min(512, math.sqrt(target / (pm(16, 1, 1) / 256 * iters)))))
+    # 256x256 for the sweep and 128x128 for PatchMatch at least, larger up to ~12 s a part per core
+    target = 12.0
+    side_s = int(max(256, min(512, math.sqrt(target / (sweep(16, 8, 1) / (256 * 8) * D)))))
+    side_p = int(max(128, min(256, math.sqrt(target / (pm(16, 1, 1) / 256 * iters)))))
     ws = sweep(side_s, D, cores)
     wp = pm(side_p, iters, cores)
     per_px = ws / (side_s * side_s) + wp / (side_p * side_p)
