@@ -14,6 +14,7 @@ struct SweepGroup {
 // the PatchMatch views of one source bucket: table entries [first, first + nv)
 struct PMGroup {
     int sm = 16, first = 0, nv = 0, blk = 0, blk0_off = 0;
+    int blk_list = 0, list0_off = 0, blk_eval = 0;  // bounded random refinement: list and evaluation grids
 };
 
 struct om_batch {
@@ -36,9 +37,11 @@ struct om_batch {
     PMGroup pm_groups[4];
     bool pm_normals = false, pm_prepare = false;
     bool pm_memo = false;         // memoised propagation (every view small enough for its item list)
+    bool pm_bounded = false;      // random refinement with the geometric bound
     int64_t pm_px = 0;
     void *pm_views = nullptr;     // PMK[pm_n], ordered by source bucket
     int *pm_blk0 = nullptr;       // per group: first block of each of its views (nv + 1 each)
+    int *pm_list0 = nullptr;      // the same for the bounded refinement's list kernel
     int *pm_row0 = nullptr;       // first row of each view (pm_n + 1)
     int64_t *pm_px0 = nullptr;    // first pixel of each view (pm_n + 1)
     unsigned long long *pm_changed = nullptr;  // (pm_n,) in the caller's view order

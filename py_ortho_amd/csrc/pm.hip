@@ -117,6 +117,14 @@ struct PMK {
     int *sorted, *bins;
     int sort_tx, sort_ty, sort_shift, sort_rx, sort_ry, sort_rshift;
     const int *sort_use;  // set for an evaluation after a sort: nonzero = read the sorted list
+    // random refinement with the geometric bound: the margin by which a photometric score can
+    // exceed 1 (+inf: no bound), the largest finite |value| of the reference and of the sources
+    // (float bits), each pixel's stream state before its current try, and the list of tries to
+    // evaluate with its two counters (one per try)
+    double *eps;
+    uint32_t *absmax;
+    uint32_t *pst;
+    int *ritems, *nritems;
 };
 
 // One source view as the cost kernels read it (device memory, uniform loads): the reference's
@@ -378,6 +386,40 @@ __device__ __forceinline__ void pm_ncc_lp(const PMK &P, int y, int x, double num
     }
 }
 
+// the geometric term of the cost (_dense.pyx:544-556): over the sources with a depth map, the
+// mean relative depth disagreement at the plane's point on the pixel's ray, each clipped to 1
+template <int SM>
+__device__ __forceinline__ double pm_geo(const PMK &P, double rpx, double rpy, double rpz, double d)
+{
+    const int S = P.S;
+    const double X = P.cr0 + d * rpx, Y = P.cr1 + d * rpy, Z = P.cr2 + d * rpz;
+    double geo = 0.0;
+    int ng = 0;
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+        const OM_CONST_AS PMCam &cs = as_const(P.cam)[s < S ? s : 0];
+        if (s < S && cs.dep) {
+            ng += 1;
+            double e = 1.0, u, v, z;
+            if (pm_proj(P, s, X, Y, Z, u, v, z)) {
+                const int ui = (int)(u + 0.5), vi = (int)(v + 0.5);
+                if (ui >= 0 && vi >= 0 && ui < cs.iw && vi < cs.ih) {
+                    const double dj = cs.dep[(size_t)vi * cs.iw + ui];
+                    if (isfinite(dj) && dj > 0) {
+                        e = fabs(z - dj) / dj / P.geo_tol;
+                        if (e > 1)
+                            e = 1;
+                    }
+                }
+            }
+            geo += e;
+        }
+    }
+    if (ng)
+        geo /= ng;
+    return geo;
+}
+
 template <int SM, int FMT>
 __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, double n1, double n2, double &photo)
 {
@@ -494,33 +536,7 @@ __device__ double pm_cost(const PMK &P, int y, int x, double d, double n0, doubl
             ph += ncc[a];
     ph /= P.kk;
     photo = ph;
-    double geo = 0.0;
-    if (P.geo_w > 0) {
-        const double X = P.cr0 + d * rpx, Y = P.cr1 + d * rpy, Z = P.cr2 + d * rpz;
-        int ng = 0;
-#pragma unroll
-        for (int s = 0; s < SM; ++s) {
-            const OM_CONST_AS PMCam &cs = as_const(P.cam)[s < S ? s : 0];
-            if (s < S && cs.dep) {
-                ng += 1;
-                double e = 1.0, u, v, z;
-                if (pm_proj(P, s, X, Y, Z, u, v, z)) {
-                    const int ui = (int)(u + 0.5), vi = (int)(v + 0.5);
-                    if (ui >= 0 && vi >= 0 && ui < cs.iw && vi < cs.ih) {
-                        const double dj = cs.dep[(size_t)vi * cs.iw + ui];
-                        if (isfinite(dj) && dj > 0) {
-                            e = fabs(z - dj) / dj / P.geo_tol;
-                            if (e > 1)
-                                e = 1;
-                        }
-                    }
-                }
-                geo += e;
-            }
-        }
-        if (ng)
-            geo /= ng;
-    }
+    const double geo = P.geo_w > 0 ? pm_geo<SM>(P, rpx, rpy, rpz, d) : 0.0;
     return (1.0 - ph) + P.geo_w * geo;
 }
 
@@ -697,6 +713,52 @@ __device__ __forceinline__ void pm_scan_view(const PMK &P, const PassK &Q)
     }
 }
 
+// a random try's plane from the pixel's plane and the stream state st, which advances past the
+// try's 4 draws; false when the reference skips the try (:648-656)
+__device__ __forceinline__ bool pm_try_plane(const PMK &P, const PassK &Q, size_t p, uint32_t &st, double &dp,
+                                             double &b0, double &b1, double &b2)
+{
+    const size_t HW = (size_t)P.H * P.W;
+    dp = (double)P.depth[p] * (1 + Q.dsc * (2 * xs_rnd(st) - 1));
+    b0 = (double)P.normal[p] + Q.nsc * (2 * xs_rnd(st) - 1);
+    b1 = (double)P.normal[HW + p] + Q.nsc * (2 * xs_rnd(st) - 1);
+    b2 = (double)P.normal[2 * HW + p] + Q.nsc * (2 * xs_rnd(st) - 1);
+    const double nn = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+    if (nn < 1e-9 || dp < P.dmin || dp > P.dmax)
+        return false;
+    b0 /= nn;
+    b1 /= nn;
+    b2 /= nn;
+    return true;
+}
+
+// both random tries of pixel p in the reference's order from stream state st (:646-664); best and
+// sc in and out -> the accepted count
+template <int SM, int FMT>
+__device__ __forceinline__ int pm_random_seq(const PMK &P, const PassK &Q, size_t p, int y, int x, uint32_t st,
+                                             double &best, double &sc)
+{
+    const size_t HW = (size_t)P.H * P.W;
+    int changed = 0;
+    double photo = -1.0;
+    for (int tr = 0; tr < 2; ++tr) {
+        double dp, b0, b1, b2;
+        if (!pm_try_plane(P, Q, p, st, dp, b0, b1, b2))
+            continue;
+        const double cst = pm_cost<SM, FMT>(P, y, x, dp, b0, b1, b2, photo);
+        if (cst < best) {
+            best = cst;
+            sc = photo;
+            P.depth[p] = (float)dp;
+            P.normal[p] = (float)b0;
+            P.normal[HW + p] = (float)b1;
+            P.normal[2 * HW + p] = (float)b2;
+            changed += 1;
+        }
+    }
+    return changed;
+}
+
 // C: the 2 random refinements with the pixel's own slice of the stream (:646-664); score write
 template <int SM, int FMT>
 __device__ __forceinline__ int pm_random_px(const PMK &P, const PassK &Q, int64_t t)
@@ -704,41 +766,210 @@ __device__ __forceinline__ int pm_random_px(const PMK &P, const PassK &Q, int64_
     int y, x;
     if (!tile_pixel(P, Q, t, y, x))
         return 0;
-    const size_t HW = (size_t)P.H * P.W, p = (size_t)y * P.W + x;
+    const size_t p = (size_t)y * P.W + x;
     if (!P.active[p])
         return 0;
     double best = P.best[p], sc = P.sc[p];
     const int rk = P.rank[p];
     int changed = 0;
-    if (rk >= 0) {
-        uint32_t st = xs_jump(P.st[Q.parity], 8ull * (unsigned long long)(P.row_off[y] + rk));
-        double photo = -1.0;
-        for (int tr = 0; tr < 2; ++tr) {
-            const double dp = (double)P.depth[p] * (1 + Q.dsc * (2 * xs_rnd(st) - 1));
-            double b0 = (double)P.normal[p] + Q.nsc * (2 * xs_rnd(st) - 1);
-            double b1 = (double)P.normal[HW + p] + Q.nsc * (2 * xs_rnd(st) - 1);
-            double b2 = (double)P.normal[2 * HW + p] + Q.nsc * (2 * xs_rnd(st) - 1);
-            const double nn = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
-            if (nn < 1e-9 || dp < P.dmin || dp > P.dmax)
-                continue;
-            b0 /= nn;
-            b1 /= nn;
-            b2 /= nn;
-            const double cst = pm_cost<SM, FMT>(P, y, x, dp, b0, b1, b2, photo);
-            if (cst < best) {
-                best = cst;
-                sc = photo;
-                P.depth[p] = (float)dp;
-                P.normal[p] = (float)b0;
-                P.normal[HW + p] = (float)b1;
-                P.normal[2 * HW + p] = (float)b2;
-                changed += 1;
-            }
-        }
-    }
+    if (rk >= 0)
+        changed = pm_random_seq<SM, FMT>(P, Q, p, y, x,
+                                         xs_jump(P.st[Q.parity], 8ull * (unsigned long long)(P.row_off[y] + rk)),
+                                         best, sc);
     P.score[p] = (float)sc;
     if (P.chg && changed)
         P.chg[p] = 1;
+    return changed;
+}
+
+// ---------------------------------------------------------------- random refinement, bounded
+// A try's cost is (1 - photo) + geo_w geo (:556).  The photometric score photo is a mean of top-k
+// NCC values cv / sqrt(vi vj) with both variances >= 1e-8 (:538-541), so it exceeds 1 only by the
+// rounding of the window sums: for windows of at most m samples and finite values of magnitude
+// <= M_I (reference) and M_J (sources), by less than 1.5 gamma_{m+3} 1e8 (M_I + M_J)^2 plus the
+// rounding of the mean, with gamma_n = n u / (1 - n u) (Higham's bound for recursive sums and
+// Cauchy-Schwarz on the exact moments).  A try whose geometric part alone, less twice that margin,
+// is not below the pixel's current cost therefore loses the reference's comparison `cst < best`
+// whatever its photometric score: it is rejected after the S projections of the geometric term.
+// The other tries are listed and evaluated one lane per try, so the lanes stay busy however few
+// remain.  Every decision, and so every output, is the reference's.
+//
+// A degenerate plane costs 10 (:474-475) and is accepted only by a pixel whose cost exceeds 10;
+// such a pixel would then take the photometric score its previous evaluation left behind, which a
+// rejected try no longer computes.  Costs above 10 need geo_w > 8 or wildly rounded NCC values, so
+// those pixels, if any, run both tries in sequence as before; every other pixel rejects degenerate
+// tries at once.  Used when geo_w > 0 (with no geometric term the bound cannot reject).
+
+// largest finite |a[i]|, merged into *out as float bits (non-negative floats order like their bits)
+__device__ __forceinline__ void pm_absmax_part(const float *a, int64_t n, uint32_t *out, int64_t i0, int64_t stride)
+{
+    float m = 0.f;
+    for (int64_t i = i0; i < n; i += stride) {
+        const float v = fabsf(a[i]);
+        if (v <= 3.402823466e38f)  // finite: NaN and inf fail
+            m = fmaxf(m, v);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.f)
+        atomicMax(out, __float_as_uint(m));
+}
+
+// block b of nb: the reference image into absmax[0], every source image into absmax[1]
+__device__ __forceinline__ void pm_absmax_view(const PMK &P, int b, int nb)
+{
+    const int64_t i0 = (int64_t)b * blockDim.x + threadIdx.x, stride = (int64_t)nb * blockDim.x;
+    pm_absmax_part(P.ref, (int64_t)P.H * P.W, P.absmax, i0, stride);
+    for (int s = 0; s < P.S; ++s) {
+        const OM_CONST_AS PMCam &cs = as_const(P.cam)[s];
+        pm_absmax_part(cs.img, (int64_t)cs.iw * cs.ih, P.absmax + 1, i0, stride);
+    }
+}
+
+// the margin: twice the most a photometric score can exceed 1, after the mean and the cost's
+// subtraction; +inf (no try rejected early) when it is too large to help
+__device__ __forceinline__ void pm_eps_view(const PMK &P)
+{
+    const double u = 0x1p-53;
+    const int nw = 2 * P.r / P.step + 1;        // window samples per side at most
+    const double m = (double)nw * nw + 4.0;
+    const double gam = m * u / (1.0 - m * u);
+    const double s = (double)__uint_as_float(P.absmax[0]) + (double)__uint_as_float(P.absmax[1]);
+    double d = 2.0 * gam * 1.0000001e8 * s * s;  // one NCC value over 1 (bound 1.5 gamma_{m+3} ...)
+    d = d + (double)(P.kk + 16) * u * (1.0 + d);  // the top-k mean, 1 - photo and the final sum
+    const double e = 2.0 * d + 1e-12;
+    *P.eps = e < 0.5 ? e : INFINITY;
+}
+
+// the workgroup's entries of a list, reserved with one atomic; -> the first entry of this lane
+// (lanes in thread order), n its entry count (every thread of the workgroup calls this)
+__device__ __forceinline__ int pm_block_reserve(int *count, int n)
+{
+    __shared__ int wbase[kPassThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o)
+            incl += v;
+    }
+    if (lane == 63)
+        wbase[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            const int c = wbase[w];
+            wbase[w] = acc;
+            acc += c;
+        }
+        const int b0 = acc ? atomicAdd(count, acc) : 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            wbase[w] += b0;
+    }
+    __syncthreads();
+    return wbase[wave] + incl - n;
+}
+
+// both tries of a pixel whose cost exceeds 10, in sequence; the pixel's second-try pass then only
+// writes its score
+template <int SM>
+__device__ __forceinline__ int pm_random_fallback(const PMK &P, const PassK &Q, size_t p)
+{
+    const int y = (int)(p / P.W), x = (int)(p - (size_t)y * P.W);
+    const uint32_t st = P.pst[p];
+    double best = P.best[p], sc = P.sc[p];
+    const int changed = pm_random_seq<SM, 0>(P, Q, p, y, x, st, best, sc);
+    P.best[p] = best;
+    P.sc[p] = sc;
+    if (P.chg && changed)
+        P.chg[p] = 1;
+    P.rank[p] = -1;
+    return changed;
+}
+
+// try tr of pixel t: its plane, the degenerate-plane test, the geometric term and the bound;
+// the try is listed unless the bound rejects it.  Pixels without a listed second try get their
+// score here; a pixel whose cost exceeds 10 is listed for both tries in sequence.
+template <int SM>
+__device__ __forceinline__ void pm_try_list_px(const PMK &P, const PassK &Q, int64_t t, int tr)
+{
+    int y, x;
+    size_t p = 0;
+    bool listed = false, fallback = false;
+    if (tile_pixel(P, Q, t, y, x)) {
+        p = (size_t)y * P.W + x;
+        if (P.active[p]) {
+            const int rk = P.rank[p];
+            if (rk >= 0) {
+                uint32_t st;
+                if (tr == 0) {
+                    st = xs_jump(P.st[Q.parity], 8ull * (unsigned long long)(P.row_off[y] + rk));
+                } else {
+                    st = P.pst[p];
+                    for (int k = 0; k < 4; ++k)
+                        (void)xs_rnd(st);
+                }
+                const double best = P.best[p];
+                if (tr == 0 && best > 10.0) {
+                    P.pst[p] = st;
+                    fallback = true;
+                } else {
+                    P.pst[p] = st;
+                    double dp, b0, b1, b2;
+                    if (pm_try_plane(P, Q, p, st, dp, b0, b1, b2)) {
+                        const size_t HW = (size_t)P.H * P.W;
+                        const double rpx = P.ray[p], rpy = P.ray[HW + p], rpz = P.ray[2 * HW + p];
+                        const double ndr = b0 * rpx + b1 * rpy + b2 * rpz;
+                        if (!(ndr >= -1e-6 || dp <= 0)) {
+                            const double g = P.geo_w * pm_geo<SM>(P, rpx, rpy, rpz, dp);
+                            listed = !(g - *P.eps >= best);
+                        }
+                    }
+                }
+            }
+            if (tr == 1 && !listed)
+                P.score[p] = (float)P.sc[p];
+        }
+    }
+    const int at = pm_block_reserve(P.nritems + tr, listed ? 1 : 0);
+    if (listed)
+        P.ritems[at] = (int)p;
+    if (tr == 0) {  // pixels with costs above 10 (in practice none), from the list's far end
+        const int af = pm_block_reserve(P.nritems + 2, fallback ? 1 : 0);
+        if (fallback)
+            P.ritems[(int64_t)P.H * ((P.W + 1) / 2) - 1 - af] = (int)p;
+    }
+}
+
+// listed try i: the full cost and the reference's comparison; -> 1 when accepted
+template <int SM>
+__device__ __forceinline__ int pm_try_eval(const PMK &P, const PassK &Q, int64_t i, int tr)
+{
+    const size_t p = (size_t)P.ritems[i];
+    const int y = (int)(p / P.W), x = (int)(p - (size_t)y * P.W);
+    const size_t HW = (size_t)P.H * P.W;
+    uint32_t st = P.pst[p];
+    double dp, b0, b1, b2;
+    pm_try_plane(P, Q, p, st, dp, b0, b1, b2);  // true: the try was listed
+    double photo = -1.0;
+    const double cst = pm_cost<SM, 0>(P, y, x, dp, b0, b1, b2, photo);
+    int changed = 0;
+    if (cst < P.best[p]) {
+        P.best[p] = cst;
+        P.sc[p] = photo;
+        P.depth[p] = (float)dp;
+        P.normal[p] = (float)b0;
+        P.normal[HW + p] = (float)b1;
+        P.normal[2 * HW + p] = (float)b2;
+        if (P.chg)
+            P.chg[p] = 1;
+        changed = 1;
+    }
+    if (tr == 1)
+        P.score[p] = (float)P.sc[p];
     return changed;
 }
 
@@ -1129,6 +1360,57 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_RANDOM_ATTR pm_random_kernel
     count_changes(P, block_total(n));
 }
 
+// bounded random refinement (single view): try tr's list, then its evaluation
+template <int SM>
+__global__ void __launch_bounds__(kPassThreads) pm_try_list_kernel(const PMK P, const PassK Q, int tr)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (b == 0 && threadIdx.x == 0) {  // the lists of the other try: their evaluation has run
+        P.nritems[1 - tr] = 0;
+        if (tr == 1)
+            P.nritems[2] = 0;
+    }
+    pm_try_list_px<SM>(P, Q, (int64_t)b * blockDim.x + threadIdx.x, tr);
+}
+
+// the pixels listed for both tries in sequence (costs above 10; in practice none)
+template <int SM>
+__global__ void __launch_bounds__(64) pm_try_fallback_kernel(const PMK P, const PassK Q)
+{
+    const int n = P.nritems[2];
+    const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
+    int c = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        c += pm_random_fallback<SM>(P, Q, (size_t)P.ritems[ncol - 1 - k]);
+    count_changes(P, c);
+}
+
+// the listed tries spread evenly over the XCDs, each XCD a contiguous range of the list
+template <int SM>
+__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_try_eval_kernel(const PMK P, const PassK Q, int tr)
+{
+    const int64_t n = P.nritems[tr];
+    const int64_t used = cdiv(n, (int64_t)blockDim.x), per = cdiv(used, (int64_t)kXCD);
+    const int xcd = blockIdx.x % kXCD, j = blockIdx.x / kXCD;
+    if (j >= per)
+        return;
+    const int64_t i = ((int64_t)xcd * per + j) * blockDim.x + threadIdx.x;
+    const int c = i < n ? pm_try_eval<SM>(P, Q, i, tr) : 0;
+    count_changes(P, block_total(c));
+}
+
+__global__ void pm_bound_reset_kernel(const PMK P)
+{
+    if (threadIdx.x < 3)
+        P.nritems[threadIdx.x] = 0;
+    if (threadIdx.x < 2)
+        P.absmax[threadIdx.x] = 0u;
+}
+
+__global__ void __launch_bounds__(256) pm_absmax_kernel(const PMK P) { pm_absmax_view(P, blockIdx.x, gridDim.x); }
+
+__global__ void pm_eps_kernel(const PMK P) { pm_eps_view(P); }
+
 __global__ void pm_init_kernel(uint32_t *st, uint32_t seed, unsigned long long *changed, const PMCamArgs A,
                                PMCam *cam)
 {
@@ -1173,6 +1455,105 @@ __global__ void __launch_bounds__(kPMThreads) OM_PM_RANDOM_ATTR pm_random_batch(
     count_changes(P, block_total(n));
 }
 
+// bounded random refinement over the views of a source bucket: try tr's lists (blocks of
+// kPassThreads threads, view v's from blk0[v]) ...
+template <int SM>
+__global__ void __launch_bounds__(kPassThreads) pm_try_list_batch(const PMK *views, const int *blk0, int nviews,
+                                                                  const PassK Q, int tr)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int v = view_of(blk0, nviews, b);
+    const PMK &P = uniform_ref(views + v);
+    const int lb = b - as_const(blk0)[v];
+    if (lb == 0 && threadIdx.x == 0) {  // the lists of the other try: their evaluation has run
+        P.nritems[1 - tr] = 0;
+        if (tr == 1)
+            P.nritems[2] = 0;
+    }
+    pm_try_list_px<SM>(P, Q, (int64_t)lb * blockDim.x + threadIdx.x, tr);
+}
+
+// the pixels listed for both tries in sequence, view blockIdx.y (costs above 10; in practice none)
+template <int SM>
+__global__ void __launch_bounds__(64) pm_try_fallback_batch(const PMK *views, const PassK Q)
+{
+    const PMK &P = uniform_ref(views + blockIdx.y);
+    const int n = P.nritems[2];
+    const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
+    int c = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        c += pm_random_fallback<SM>(P, Q, (size_t)P.ritems[ncol - 1 - k]);
+    count_changes(P, c);
+}
+
+// ... and their evaluation: the views' listed tries in blocks of kPMThreads as one sequence (view
+// order), each XCD a contiguous range of it; a block finds its view from the list counts
+template <int SM>
+__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_try_eval_batch(const PMK *views, int nviews,
+                                                                           const PassK Q, int tr)
+{
+    __shared__ int s_v, s_lb;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int total = 0;
+        for (int v0 = 0; v0 < nviews; v0 += 64) {
+            const int v = v0 + lane;
+            int u = v < nviews ? (int)cdiv((int64_t)views[v].nritems[tr], (int64_t)kPMThreads) : 0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+                u += __shfl_xor(u, o, 64);
+            total += u;
+        }
+        const int per = (total + kXCD - 1) / kXCD;
+        const int xcd = blockIdx.x % kXCD, j = blockIdx.x / kXCD;
+        const int L = j < per ? xcd * per + j : total;  // this block's place in the sequence
+        int fv = -1, fl = 0, acc = 0;
+        for (int v0 = 0; v0 < nviews && fv < 0 && L < total; v0 += 64) {
+            const int v = v0 + lane;
+            const int u = v < nviews ? (int)cdiv((int64_t)views[v].nritems[tr], (int64_t)kPMThreads) : 0;
+            int incl = u;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int w = __shfl_up(incl, o, 64);
+                if (lane >= o)
+                    incl += w;
+            }
+            const int start = acc + incl - u;
+            const unsigned long long m = __ballot(u > 0 && L >= start && L < start + u);
+            if (m) {
+                const int l = __builtin_ctzll(m);
+                fv = v0 + l;
+                fl = L - __shfl(start, l, 64);
+            }
+            acc += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) {
+            s_v = fv;
+            s_lb = fl;
+        }
+    }
+    __syncthreads();
+    const int v = __builtin_amdgcn_readfirstlane(s_v);
+    if (v < 0)
+        return;
+    const PMK &P = uniform_ref(views + v);
+    const int64_t i = (int64_t)s_lb * blockDim.x + threadIdx.x;
+    const int c = i < (int64_t)P.nritems[tr] ? pm_try_eval<SM>(P, Q, i, tr) : 0;
+    count_changes(P, block_total(c));
+}
+
+__global__ void __launch_bounds__(256) pm_absmax_batch(const PMK *views)
+{
+    pm_absmax_view(uniform_ref(views + blockIdx.y), blockIdx.x, gridDim.x);
+}
+
+__global__ void pm_eps_batch(const PMK *views, int nviews)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nviews)
+        pm_eps_view(views[v]);
+}
+
 // item counters of every view; with clear_chg also the evaluation counts and the changed flags of
 // every pixel (px0: first pixel of each view)
 __global__ void pm_memo_reset_batch(const PMK *views, const int64_t *px0, int nviews, int clear_chg)
@@ -1214,6 +1595,8 @@ __global__ void pm_init_batch(const PMK *views, int nviews)
     if (v < nviews) {
         views[v].st[0] = (uint32_t)views[v].seed | 1u;
         *views[v].changed = 0ull;
+        views[v].nritems[0] = views[v].nritems[1] = views[v].nritems[2] = 0;
+        views[v].absmax[0] = views[v].absmax[1] = 0u;
     }
 }
 
@@ -1229,6 +1612,9 @@ struct PMWs {
     uint8_t *chg;  // (H, W)
     int *items, *nitems;
     unsigned long long *nevals;
+    double *eps;   // bounded random refinement: margin, image maxima, stream states, try list
+    uint32_t *absmax, *pst;
+    int *ritems, *nritems;
 };
 
 // The memo path lists items as pixel * 16 + slot in an int, so it serves images below 2^27 pixels;
@@ -1263,6 +1649,14 @@ static void pm_sort_tiles(PMK &P, int iw0, int ih0)
     P.sort_ry = (int)cdiv(P.H, 1 << rs);
 }
 
+// The random refinement rejects tries by the geometric bound unless OM_PM_RANDOM_BOUND=0
+// (environment, read at each call; the tests use it to run the sequential kernel on the same inputs).
+static bool pm_bound_on()
+{
+    const char *e = getenv("OM_PM_RANDOM_BOUND");
+    return !(e && e[0] == '0');
+}
+
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
@@ -1276,6 +1670,8 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
     };
     char *b = take(HW * 8), *s = take(HW * 8), *rk = take(HW * 4), *rc = take((size_t)H * 4),
          *ro = take((size_t)H * 4), *st = take(16), *ch = take(8), *cm = take(sizeof(PMCam) * 16);
+    char *ep = take(8), *am = take(8), *ps = take(HW * 4), *ri = take((size_t)H * ((W + 1) / 2) * 4),
+         *nr = take(16);
     char *cc = nullptr, *cg = nullptr, *it = nullptr, *ni = nullptr, *ne = nullptr, *so = nullptr, *bn = nullptr;
     if (memo) {
         const size_t ncol = (size_t)H * ((W + 1) / 2);
@@ -1303,6 +1699,11 @@ static size_t pm_ws_layout(int H, int W, char *base, PMWs *w, bool memo = false)
         w->st = (uint32_t *)st;
         w->changed = (unsigned long long *)ch;
         w->cam = (PMCam *)cm;
+        w->eps = (double *)ep;
+        w->absmax = (uint32_t *)am;
+        w->pst = (uint32_t *)ps;
+        w->ritems = (int *)ri;
+        w->nritems = (int *)nr;
     }
     return o;
 }
@@ -1324,13 +1725,26 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
     P.nevals = w.nevals;
     P.sorted = w.sorted;
     P.bins = w.bins;
+    P.eps = w.eps;
+    P.absmax = w.absmax;
+    P.pst = w.pst;
+    P.ritems = w.ritems;
+    P.nritems = w.nritems;
     pm_sort_tiles(P, A.cam[0].iw, A.cam[0].ih);
     const int64_t HW = (int64_t)P.H * P.W;
     // one block per 128 possible items, rounded up to whole rounds over the XCDs
     const int64_t eval_grid = cdiv(cdiv(9 * ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD;
+    const bool bounded = FMT == 0 && P.geo_w > 0 && pm_bound_on();
+    const unsigned gL = (unsigned)cdiv(need_threads(P.H, P.W), kPassThreads);
+    const unsigned gE = (unsigned)(cdiv(cdiv(ncol, (int64_t)kPMThreads), (int64_t)kXCD) * kXCD);
     hipLaunchKernelGGL(pm_init_kernel, dim3(1), dim3(1), 0, stream, w.st, seed, w.changed, A, w.cam);
     if (memo)
         hipLaunchKernelGGL(pm_memo_reset_kernel, dim3((unsigned)cdiv(HW, 256)), dim3(256), 0, stream, P, HW, 1);
+    if (bounded) {
+        hipLaunchKernelGGL(pm_bound_reset_kernel, dim3(1), dim3(64), 0, stream, P);
+        hipLaunchKernelGGL(pm_absmax_kernel, dim3(256), dim3(256), 0, stream, P);
+        hipLaunchKernelGGL(pm_eps_kernel, dim3(1), dim3(1), 0, stream, P);
+    }
     int pass = 0;
     for (int it = 0; it < iters; ++it) {
         for (int colour = 0; colour < 2; ++colour, ++pass) {
@@ -1366,7 +1780,16 @@ static int pm_run(PMK &P, const PMCamArgs &A, int iters, hipStream_t stream, uin
             }
             {
                 ProfScope ps("pm_random", stream);
-                hipLaunchKernelGGL((pm_random_kernel<SM, FMT>), dim3(gR), dim3(kPMThreads), 0, stream, P, Q);
+                if (bounded) {
+                    for (int tr = 0; tr < 2; ++tr) {
+                        hipLaunchKernelGGL(pm_try_list_kernel<SM>, dim3(gL), dim3(kPassThreads), 0, stream, P, Q, tr);
+                        hipLaunchKernelGGL(pm_try_eval_kernel<SM>, dim3(gE), dim3(kPMThreads), 0, stream, P, Q, tr);
+                        if (tr == 0)
+                            hipLaunchKernelGGL(pm_try_fallback_kernel<SM>, dim3(4), dim3(64), 0, stream, P, Q);
+                    }
+                } else {
+                    hipLaunchKernelGGL((pm_random_kernel<SM, FMT>), dim3(gR), dim3(kPMThreads), 0, stream, P, Q);
+                }
             }
         }
     }
@@ -1721,6 +2144,11 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
         Q.nevals = w.nevals;
         Q.sorted = w.sorted;
         Q.bins = w.bins;
+        Q.eps = w.eps;
+        Q.absmax = w.absmax;
+        Q.pst = w.pst;
+        Q.ritems = w.ritems;
+        Q.nritems = w.nritems;
         pm_sort_tiles(Q, V.iw[0], V.ih[0]);
         if (memo)
             b->pm_nevals[v] = w.nevals;
@@ -1733,7 +2161,7 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int c) { return sm_bucket(P[a].S) < sm_bucket(P[c].S); });
     std::vector<PMK> Ps(nviews);
-    std::vector<int> row0(nviews + 1, 0), gblk0;
+    std::vector<int> row0(nviews + 1, 0), gblk0, glist0;
     std::vector<int64_t> px0(nviews + 1, 0);
     b->pm_ngroups = 0;
     for (int i = 0; i < nviews; ++i) {
@@ -1750,21 +2178,33 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
             g.blk = 0;
             g.blk0_off = (int)gblk0.size();
             gblk0.push_back(0);
+            g.blk_list = 0;
+            g.blk_eval = 0;
+            g.list0_off = (int)glist0.size();
+            glist0.push_back(0);
         }
         PMGroup &g = b->pm_groups[b->pm_ngroups - 1];
         g.blk += (int)cdiv(need_threads(Q.H, Q.W), kPMThreads);  // random refinement walks tiles
         g.nv += 1;
         gblk0.push_back(g.blk);
+        g.blk_list += (int)cdiv(need_threads(Q.H, Q.W), kPassThreads);
+        glist0.push_back(g.blk_list);
+        g.blk_eval += (int)cdiv((int64_t)Q.H * ((Q.W + 1) / 2), (int64_t)kPMThreads);
     }
+    for (int gi = 0; gi < b->pm_ngroups; ++gi)  // whole rounds over the XCDs
+        b->pm_groups[gi].blk_eval = (int)(cdiv((int64_t)b->pm_groups[gi].blk_eval, (int64_t)kXCD) * kXCD);
     b->pm_views = b->dev_alloc(sizeof(PMK) * nviews);
     b->pm_blk0 = (int *)b->dev_alloc(sizeof(int) * gblk0.size());
+    b->pm_list0 = (int *)b->dev_alloc(sizeof(int) * glist0.size());
     b->pm_row0 = (int *)b->dev_alloc(sizeof(int) * (nviews + 1));
     b->pm_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
-    if (!b->pm_views || !b->pm_blk0 || !b->pm_row0 || !b->pm_px0)
+    if (!b->pm_views || !b->pm_blk0 || !b->pm_list0 || !b->pm_row0 || !b->pm_px0)
         return fail(OM_ENOMEM, "batch: PatchMatch tables");
     int rc;
     if ((rc = hip_check(hipMemcpy(b->pm_views, Ps.data(), sizeof(PMK) * nviews, hipMemcpyHostToDevice), "upload")) ||
         (rc = hip_check(hipMemcpy(b->pm_blk0, gblk0.data(), sizeof(int) * gblk0.size(), hipMemcpyHostToDevice),
+                        "upload")) ||
+        (rc = hip_check(hipMemcpy(b->pm_list0, glist0.data(), sizeof(int) * glist0.size(), hipMemcpyHostToDevice),
                         "upload")) ||
         (rc = hip_check(hipMemcpy(b->pm_row0, row0.data(), sizeof(int) * (nviews + 1), hipMemcpyHostToDevice),
                         "upload")) ||
@@ -1778,6 +2218,7 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
     b->pm_px = px0[nviews];
     b->pm_normals = init_normals != 0;
     b->pm_memo = memo;
+    b->pm_bounded = geo_w > 0 && pm_bound_on();
     b->pm_prepare = b->pm_normals;
     for (int v = 0; v < nviews; ++v)
         b->pm_prepare = b->pm_prepare || views[v].depth_from || views[v].score_from;
@@ -1829,6 +2270,29 @@ static void pm_group_launch(const om_batch *b, const PMGroup &g, bool propagate,
 #undef OM_PM_GROUP_CASE
 }
 
+// the bounded random refinement of one source bucket: try tr's lists, then their evaluation
+static void pm_group_try(const om_batch *b, const PMGroup &g, int tr, const PassK &Q, hipStream_t st)
+{
+    const PMK *V = (const PMK *)b->pm_views + g.first;
+    const int *list0 = b->pm_list0 + g.list0_off;
+#define OM_PM_TRY_CASE(SM_)                                                                                            \
+    case SM_:                                                                                                          \
+        hipLaunchKernelGGL(pm_try_list_batch<SM_>, dim3((unsigned)g.blk_list), dim3(kPassThreads), 0, st, V, list0,    \
+                           g.nv, Q, tr);                                                                               \
+        hipLaunchKernelGGL(pm_try_eval_batch<SM_>, dim3((unsigned)g.blk_eval), dim3(kPMThreads), 0, st, V, g.nv, Q,    \
+                           tr);                                                                                        \
+        if (tr == 0)                                                                                                   \
+            hipLaunchKernelGGL(pm_try_fallback_batch<SM_>, dim3(4, (unsigned)g.nv), dim3(64), 0, st, V, Q);            \
+        break;
+    switch (g.sm) {
+        OM_PM_TRY_CASE(4)
+        OM_PM_TRY_CASE(8)
+        OM_PM_TRY_CASE(12)
+        OM_PM_TRY_CASE(16)
+    }
+#undef OM_PM_TRY_CASE
+}
+
 static void pm_batch_passes(om_batch *b, hipStream_t st)
 {
     const PMK *V = (const PMK *)b->pm_views;
@@ -1859,8 +2323,14 @@ static void pm_batch_passes(om_batch *b, hipStream_t st)
             hipLaunchKernelGGL(pm_scan_rows_batch, dim3((unsigned)b->pm_n), dim3(256), 0, st, V, Q);
             {
                 ProfScope ps("pm_random_batch", st);
-                for (int gi = 0; gi < b->pm_ngroups; ++gi)
-                    pm_group_launch(b, b->pm_groups[gi], false, Q, st);
+                for (int gi = 0; gi < b->pm_ngroups; ++gi) {
+                    if (b->pm_bounded) {
+                        pm_group_try(b, b->pm_groups[gi], 0, Q, st);
+                        pm_group_try(b, b->pm_groups[gi], 1, Q, st);
+                    } else {
+                        pm_group_launch(b, b->pm_groups[gi], false, Q, st);
+                    }
+                }
             }
         }
     }
@@ -1879,6 +2349,10 @@ int batch_pm_enqueue(om_batch *b, hipStream_t st)
     if (b->pm_memo)
         hipLaunchKernelGGL(pm_memo_reset_batch, dim3((unsigned)cdiv(b->pm_px > b->pm_n ? b->pm_px : b->pm_n, 256)),
                            dim3(256), 0, st, V, b->pm_px0, b->pm_n, 1);
+    if (b->pm_bounded) {  // the margin of every view from its images' largest values
+        hipLaunchKernelGGL(pm_absmax_batch, dim3(64, (unsigned)b->pm_n), dim3(256), 0, st, V);
+        hipLaunchKernelGGL(pm_eps_batch, dim3((unsigned)cdiv(b->pm_n, 64)), dim3(64), 0, st, V, b->pm_n);
+    }
     pm_batch_passes(b, st);
     return launch_check("pm batch kernels");
 }
