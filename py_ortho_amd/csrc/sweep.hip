@@ -163,13 +163,19 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const int H = P.H, W = P.W;
     const size_t HW = (size_t)H * W;
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
-    // image pixel of halo pixel hp, -1 outside the image (or in the pad column)
-    auto halo_px = [&](int hp) -> int {
-        const int hy = hp / HCP, hx = hp - hy * HCP;
+    // image pixel of the halo pixel at row hy, column hx, -1 outside the image (or in the pad column)
+    auto halo_at = [&](int hy, int hx) -> int {
         const int gy = ty0 - r + hy, gx = tx0 - r + hx;
         return (hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W) ? gy * W + gx : -1;
     };
-    auto pix = [&](int hp) -> int { return CD ? halo_px(hp) : sP[hp]; };
+    auto halo_px = [&](int hp) -> int {
+        const int hy = hp / HCP;
+        return halo_at(hy, hp - hy * HCP);
+    };
+    // a thread's halo pixels are tid, tid + kThreads, ...: its first one's row and column, and the
+    // step between them (the warp pass walks them without a division per pixel)
+    const int hy_t = tid / HCP, hx_t = tid - hy_t * HCP;
+    const int hy_step = kThreads / HCP, hx_step = kThreads - hy_step * HCP;
 
     // ---- the tile + halo: pixel index (unless recomputed) and reference value
     for (int hp = tid; hp < NHP; hp += kThreads) {
@@ -258,7 +264,8 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             // works on its own.
             const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
                        *rayz = (const char *)(P.ray + 2 * HW);
-            int pn = tid < NHP ? pix(tid) : -1;
+            int hyn = hy_t, hxn = hx_t;
+            int pn = tid < NHP ? (CD ? halo_at(hyn, hxn) : sP[tid]) : -1;
             uint32_t on = (uint32_t)max(pn, 0) << 2;
             float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
                   rzn = *(const float *)(rayz + on);
@@ -269,7 +276,13 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     J[gg] = NAN;
                 const int p = pn;
                 const double rx = rxn, ry = ryn, rz = rzn;
-                pn = hp + kThreads < NHP ? pix(hp + kThreads) : -1;
+                hyn += hy_step;
+                hxn += hx_step;
+                if (hxn >= HCP) {
+                    hxn -= HCP;
+                    ++hyn;
+                }
+                pn = hp + kThreads < NHP ? (CD ? halo_at(hyn, hxn) : sP[hp + kThreads]) : -1;
                 on = (uint32_t)max(pn, 0) << 2;
                 rxn = *(const float *)(rayx + on);
                 ryn = *(const float *)(rayy + on);
