@@ -133,7 +133,10 @@ int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *stream);
  * ray (3,H,W), depth (H,W), normal (3,H,W), score (H,W) in/out, active (H,W) uint8; imgs/deps:
  * S device pointers (deps[s] may be NULL), ih/iw/par (S,18)/Cr(3): host.  *changed receives the
  * number of accepted hypothesis changes (the reference's return value); it synchronises `stream`
- * when changed != NULL.  workspace: om_pm_workspace_bytes(H, W) bytes of device memory. */
+ * when changed != NULL.  workspace: om_pm_workspace_bytes(H, W) bytes of device memory.
+ * With geo_w > 0 a random try whose geometric cost alone (less a bound on the photometric score's
+ * rounding above 1) is not below the pixel's cost is rejected without its photometric part: the
+ * reference's comparison rejects it whatever that part is, so the results are the same. */
 size_t om_pm_workspace_bytes(int32_t H, int32_t W);
 int om_pm_refine(const float *ref, int32_t H, int32_t W, const float *ray, const double *Cr, int32_t S,
                  const float *const *imgs, const float *const *deps, const int32_t *ih, const int32_t *iw,
