@@ -779,7 +779,8 @@ def cpu_sweep_rate(ref, mu, sd, ray, srcs, geo, planes, window, top_k, args, wha
 # ------------------------------------------------------------------------------- C4 precision variant
 def run_c4(args, torch):
     """4000x3000 reference, S = 12, 384 planes + parabola + 8 PatchMatch iterations, with FP16 or BF16
-    image storage (--fmt); the same work in FP64 is run once beside it to state the error."""
+    image storage (--fmt); the same work in FP64, and with FP32 storage (the variant's FP32 arithmetic
+    without the 16-bit rounding), is run once beside it to state the error against both."""
     from py_ortho_amd import _lib, batch, block, dense, scene
     lib = _lib.lib()
     t_scene = time.perf_counter()
@@ -799,25 +800,29 @@ def run_c4(args, torch):
     deps = [v.depth.contiguous() for v in sc.srcs]
     active = torch.ones((H, W), dtype=torch.uint8, device="cuda")
     fmt = args.fmt
-    ref16 = dense.to_lp(ref, fmt)
-    srcs16 = [dense.to_lp(x, fmt) for x in srcs]
+    stored = {}  # the images in each storage format, converted once (inputs resident in HBM)
 
-    def step(lp: bool):
+    def step(f):
+        """the whole view with image storage f ("f16", "bf16", "f32") or in FP64 (None)"""
         st = dense.SweepState(H, W)
-        if lp:
-            dense.sweep_lp(ref16, mu, sd, srcs16, ray, geo, hyp, r, k, fmt, state=st)
+        if f:
+            if f not in stored:
+                stored[f] = (dense.to_lp(ref, f), [dense.to_lp(x, f) for x in srcs])
+            ref_f, srcs_f = stored[f]
+            dense.sweep_lp(ref_f, mu, sd, srcs_f, ray, geo, hyp, r, k, f, state=st)
         else:
+            srcs_f = srcs
             dense.sweep(ref, mu, sd, srcs, ray, geo, hyp, r, k, state=st)
         d = dense.sweep_finalize(ref, mu, sd, srcs, ray, geo, hyp, r, k, st)
         d_pm = d.clone()
         nrm = batch.init_normals(d, ray)
         score = st.best.clone()
-        dense.pm_refine(ref, ray, sc.ref.cam.C, srcs16 if lp else srcs, deps, sc.par(), d_pm, nrm, score, active,
-                        pr, pstep, k, 0.5, 0.01, iters, dmin, dmax, 2654435761, fmt=fmt if lp else None)
+        dense.pm_refine(ref, ray, sc.ref.cam.C, srcs_f, deps, sc.par(), d_pm, nrm, score, active,
+                        pr, pstep, k, 0.5, 0.01, iters, dmin, dmax, 2654435761, fmt=f)
         return st, d, d_pm
 
     for _ in range(args.warmup):
-        step(True)
+        step(fmt)
     torch.cuda.synchronize()
     lib.om_profile_enable(1)
     prof_read(lib, "sweep_lp")
@@ -825,7 +830,7 @@ def run_c4(args, torch):
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        st_lp, d_lp, pm_lp = step(True)
+        st_lp, d_lp, pm_lp = step(fmt)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - ts) * 1e3)
     wall = time.perf_counter() - t0
@@ -836,15 +841,33 @@ def run_c4(args, torch):
     ms = wall / args.steps * 1e3
     torch.cuda.synchronize()
     t64 = time.perf_counter()
-    st64, d64, pm64 = step(False)
+    st64, d64, pm64 = step(None)
     torch.cuda.synchronize()
     t64 = (time.perf_counter() - t64) * 1e3
     valid = (st64.best > 0.3) & (64.0 * sd > 0.5)
-    same = float((st_lp.idx == st64.idx)[valid].float().mean())
-    db = (st_lp.best - st64.best).abs()[valid]
-    rel_sw = ((d_lp - d64).abs() / d64)[valid]
     both = torch.isfinite(pm_lp) & torch.isfinite(pm64)
-    rel_pm = ((pm_lp - pm64).abs() / pm64)[both]
+
+    def errors(a, b):
+        """per-pixel cost and depth error of run a against run b (valid pixels of the FP64 sweep)"""
+        (st_a, d_a, pm_a), (st_b, d_b, pm_b) = a, b
+        db = (st_a.best - st_b.best).abs()[valid]
+        rel_sw = ((d_a - d_b).abs() / d_b)[valid]
+        fin = torch.isfinite(pm_a) & torch.isfinite(pm_b)
+        rel_pm = ((pm_a - pm_b).abs() / pm_b)[fin]
+        sub = lambda t: t.float()[:: max(1, t.numel() // 1000000)]  # noqa: E731
+        return {"sweep_same_winner": float((st_a.idx == st_b.idx)[valid].float().mean()),
+                "sweep_abs_best_median": float(db.median()), "sweep_abs_best_p99": float(torch.quantile(sub(db), 0.99)),
+                "sweep_depth_rel_p99": float(torch.quantile(sub(rel_sw), 0.99)),
+                "pm_depth_within_0.2pct": float((rel_pm < 2e-3).float().mean())}
+
+    run_lp, run64 = (st_lp, d_lp, pm_lp), (st64, d64, pm64)
+    err64 = errors(run_lp, run64)
+    if fmt != "f32":
+        run32 = step("f32")
+        err32 = errors(run_lp, run32)
+        err32_64 = errors(run32, run64)
+    else:
+        err32, err32_64 = None, err64
     truth = sc.ref.depth
     px_h = H * W * (D + 11 * iters)
     flops = H * W * D * sweep_flops_per_hyp(S)
@@ -860,12 +883,7 @@ def run_c4(args, torch):
                    "scene_s": t_scene},
         "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
         "fp64_same_work_ms": t64, "speedup_vs_fp64": t64 / ms,
-        "error_vs_fp64": {"sweep_same_winner": same, "sweep_abs_best_median": float(db.median()),
-                          "sweep_abs_best_p99": float(torch.quantile(db.float()[:: max(1, db.numel() // 1000000)],
-                                                                     0.99)),
-                          "sweep_depth_rel_p99": float(torch.quantile(rel_sw.float()[:: max(1, rel_sw.numel()
-                                                                                              // 1000000)], 0.99)),
-                          "pm_depth_within_0.2pct": float((rel_pm < 2e-3).float().mean())},
+        "error_vs_fp64": err64, "error_vs_f32_storage": err32, "f32_storage_error_vs_fp64": err32_64,
         "accuracy_vs_truth": {"pm_median_rel_err": float((((pm_lp - truth).abs() / truth)[both]).median()),
                               "pm_fp64_median_rel_err": float((((pm64 - truth).abs() / truth)[both]).median())},
         "roofline": {"bound": "valu", "engine": "VALU FP32 (no MFMA: gather-and-reduce)",
@@ -1226,7 +1244,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fuse", "sgm", "prep"], default="c3")
     ap.add_argument("--sgm-tiles", type=int, default=256, help="sgm: tiles aggregated in one call")
-    ap.add_argument("--fmt", choices=["f16", "bf16"], default="f16", help="C4: image storage format")
+    ap.add_argument("--fmt", choices=["f16", "bf16", "f32"], default="f16", help="C4: image storage format")
     ap.add_argument("--rate", type=float, default=40.0, help="C5: Poisson arrival rate (requests/s)")
     ap.add_argument("--duration", type=float, default=3.0, help="C5: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="C5: admission budget (GB of HBM)")

@@ -238,16 +238,19 @@ size_t om_sweep_multi_bytes(int32_t nviews);
 int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float *const *depth_out, int32_t tile_h, void *table,
                    size_t table_bytes, void *staging, void *stream);
 
-/* Reduced-precision storage formats of om_sweep_lp / om_to_lp */
+/* Storage formats of the precision variants (om_sweep_lp, om_pm_refine_lp) and om_to_lp: FP16,
+ * BF16, or FP32 (the images as they are: the variants' FP32 arithmetic without the 16-bit rounding,
+ * the baseline their storage error is stated against) */
 #define OM_FMT_F16 1
 #define OM_FMT_BF16 2
+#define OM_FMT_F32 3
 
-/* float32 -> FP16 or BF16 (round to nearest even, NaN kept), n elements */
+/* float32 -> FP16 or BF16 (round to nearest even, NaN kept), or a copy for FP32, n elements */
 int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream);
 
 /* The precision variant of om_sweep (no reference counterpart: the reference computes in FP64 on
- * the CPU, _dense.pyx:740-838): reference (ref16) and source images (src[s].img) stored in FP16 or
- * BF16, warp and bilinear in FP32, window sums and NCC in FP32, plane depth and the variance guard in
+ * the CPU, _dense.pyx:740-838): reference (ref16) and source images (src[s].img) stored in FP16,
+ * BF16 or FP32 (fmt), warp and bilinear in FP32, window sums and NCC in FP32, plane depth and the variance guard in
  * FP64.  Hypotheses [i0, i0 + nh) continue the desc's winner state (one chunk); the radial model
  * (k1..k3) is evaluated in FP32; the state is finished by om_sweep_finalize.  Errors against the FP64 path are stated in
  * the tests and benchmark. */
@@ -259,7 +262,7 @@ int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref16, void *st
 int om_pm_workspace_counts(const void *workspace, int32_t H, int32_t W, int64_t *evaluated);
 
 /* The precision variant of om_pm_refine (no reference counterpart: the reference's cost is FP64,
- * _dense.pyx:461-556): source images imgs16[s] stored as FP16 or BF16 (fmt), the plane (num = d n.ray)
+ * _dense.pyx:461-556): source images imgs16[s] stored as FP16, BF16 or FP32 (fmt), the plane (num = d n.ray)
  * and the geometric term in FP64, per sample the ray-plane intersection, projection and bilinear tap
  * in FP32, window sums in FP32, the variances and their guard in FP64.  Same red-black order and random
  * stream as om_pm_refine; the radial model (k1..k3) in FP32; same workspace. */

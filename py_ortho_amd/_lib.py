@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libom355.so")
 
 OM_OK, OM_EINVAL, OM_ESHAPE, OM_ENOMEM, OM_EHIP, OM_ENSRC, OM_EUNSUP = 0, -1, -2, -3, -4, -5, -6
 OM_HYP_MAPS, OM_HYP_LINEAR, OM_HYP_REFINE, OM_HYP_CONST = 0, 1, 2, 3
-OM_FMT_F16, OM_FMT_BF16 = 1, 2
+OM_FMT_F16, OM_FMT_BF16, OM_FMT_F32 = 1, 2, 3
 
 c_void_p = ctypes.c_void_p
 c_int32 = ctypes.c_int32

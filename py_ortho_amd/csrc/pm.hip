@@ -222,10 +222,13 @@ __device__ __forceinline__ double *pm_ncc_scratch()
 }
 
 // slanted-window cost (_dense.pyx:461-556); photo is written only on the full path
+// element i of a source image stored in format FMT (precision variant), as float
 template <int FMT>
-__device__ __forceinline__ float pm_ld16(const uint16_t *p)
+__device__ __forceinline__ float pm_ld_img(const void *base, size_t i)
 {
-    const uint16_t v = *p;
+    if (FMT == OM_FMT_F32)
+        return ((const float *)base)[i];
+    const uint16_t v = ((const uint16_t *)base)[i];
     if (FMT == OM_FMT_BF16)
         return __uint_as_float((uint32_t)v << 16);
     return (float)__builtin_bit_cast(_Float16, v);
@@ -347,9 +350,10 @@ __device__ __forceinline__ void pm_ncc_lp(const PMK &P, int y, int x, double num
                         if (pu >= 0.f && pv >= 0.f && pu <= cs.wlf && pv <= cs.hlf) {
                             const int x0 = (int)pu, y0 = (int)pv;
                             const float fx = pu - (float)x0, fy = pv - (float)y0;
-                            const uint16_t *pp = (const uint16_t *)cs.img + (size_t)y0 * cs.iw + x0;
-                            const float a00 = pm_ld16<FMT>(pp), a01 = pm_ld16<FMT>(pp + 1);
-                            const float a10 = pm_ld16<FMT>(pp + cs.iw), a11 = pm_ld16<FMT>(pp + cs.iw + 1);
+                            const size_t o = (size_t)y0 * cs.iw + x0;
+                            const float a00 = pm_ld_img<FMT>(cs.img, o), a01 = pm_ld_img<FMT>(cs.img, o + 1);
+                            const float a10 = pm_ld_img<FMT>(cs.img, o + cs.iw),
+                                        a11 = pm_ld_img<FMT>(cs.img, o + cs.iw + 1);
                             const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
                             const float J = fmaf(fy, t1 - t0, t0);
                             if (isfinite(J)) {
@@ -1900,11 +1904,13 @@ static int pm_refine_impl(const float *ref, int32_t H, int32_t W, const float *r
     hipStream_t st = as_stream(stream);
     int rc;
     using RunFn = int (*)(PMK &, const PMCamArgs &, int, hipStream_t, uint32_t, const PMWs &);
-    static const RunFn runs[3][4] = {{pm_run<4, 0>, pm_run<8, 0>, pm_run<12, 0>, pm_run<16, 0>},
+    static const RunFn runs[4][4] = {{pm_run<4, 0>, pm_run<8, 0>, pm_run<12, 0>, pm_run<16, 0>},
                                      {pm_run<4, OM_FMT_F16>, pm_run<8, OM_FMT_F16>, pm_run<12, OM_FMT_F16>,
                                       pm_run<16, OM_FMT_F16>},
                                      {pm_run<4, OM_FMT_BF16>, pm_run<8, OM_FMT_BF16>, pm_run<12, OM_FMT_BF16>,
-                                      pm_run<16, OM_FMT_BF16>}};
+                                      pm_run<16, OM_FMT_BF16>},
+                                     {pm_run<4, OM_FMT_F32>, pm_run<8, OM_FMT_F32>, pm_run<12, OM_FMT_F32>,
+                                      pm_run<16, OM_FMT_F32>}};
     const int bucket = S <= 4 ? 0 : (S <= 8 ? 1 : (S <= 12 ? 2 : 3));
     rc = runs[fmt][bucket](P, A, iters, st, seed, w);
     if (rc)
@@ -1955,7 +1961,7 @@ extern "C" int om_pm_refine_lp(const float *ref, int32_t H, int32_t W, const flo
                                double geo_tol, int32_t iters, double dmin, double dmax, uint32_t seed, void *workspace,
                                size_t workspace_bytes, int64_t *changed, int32_t fmt, void *stream)
 {
-    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16)
+    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16 && fmt != OM_FMT_F32)
         return fail(OM_EINVAL, "unknown storage format %d", fmt);
     return pm_refine_impl(ref, H, W, ray, Cr, S, imgs16, deps, ih, iw, par, depth, normal, score, active, r, step,
                           top_k, geo_w, geo_tol, iters, dmin, dmax, seed, workspace, workspace_bytes, changed, stream,

@@ -34,7 +34,7 @@ constexpr int kRun = 8;        // columns per horizontal running sum
 constexpr int kMaxR = 7;
 
 struct SrcLP {
-    const uint16_t *img;
+    const void *img;          // FP16 / BF16 patterns or FP32 values (FMT)
     int h, w;
     float k1, k2, k3;  // radial distortion (0 for a pinhole)
     float g[12];  // Rs (9, row-major) and b = Rs (C_ref - C_src) (3), rounded from FP64
@@ -42,7 +42,7 @@ struct SrcLP {
 };
 
 struct SweepLPK {
-    const uint16_t *ref;
+    const void *ref;
     const float *mu, *sd, *ray;
     int H, W, n, r, k, mode;
     const float *ha, *hb;
@@ -53,10 +53,13 @@ struct SweepLPK {
     SrcLP src[16];
 };
 
+// element i of an image stored in format FMT, as float
 template <int FMT>
-__device__ __forceinline__ float ld16(const uint16_t *p)
+__device__ __forceinline__ float ld_img(const void *base, size_t i)
 {
-    const uint16_t v = *p;
+    if (FMT == OM_FMT_F32)
+        return ((const float *)base)[i];
+    const uint16_t v = ((const uint16_t *)base)[i];
     if (FMT == OM_FMT_BF16)
         return __uint_as_float((uint32_t)v << 16);
     return (float)__builtin_bit_cast(_Float16, v);
@@ -124,7 +127,7 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
         const bool in = hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W;
         const int p = in ? gy * W + gx : -1;
         sP[o] = p;
-        sI[o] = in ? ld16<FMT>(P.ref + p) : 0.f;
+        sI[o] = in ? ld_img<FMT>(P.ref, (size_t)p) : 0.f;
     }
 
     const int vc = tid % kTW, vy0 = (tid / kTW) * kVR;
@@ -208,9 +211,9 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                         if (u >= 0.f && v >= 0.f && u <= S.wl && v <= S.hl) {
                             const int x0 = (int)u, y0 = (int)v;
                             const float fx = u - (float)x0, fy = v - (float)y0;
-                            const uint16_t *pp = S.img + (size_t)y0 * S.w + x0;
-                            const float a00 = ld16<FMT>(pp), a01 = ld16<FMT>(pp + 1);
-                            const float a10 = ld16<FMT>(pp + S.w), a11 = ld16<FMT>(pp + S.w + 1);
+                            const size_t o = (size_t)y0 * S.w + x0;
+                            const float a00 = ld_img<FMT>(S.img, o), a01 = ld_img<FMT>(S.img, o + 1);
+                            const float a10 = ld_img<FMT>(S.img, o + S.w), a11 = ld_img<FMT>(S.img, o + S.w + 1);
                             const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
                             const float val = fmaf(fy, t1 - t0, t0);
                             if (isfinite(val))
@@ -325,12 +328,17 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     }
 }
 
-__global__ void to_lp_kernel(const float *src, int64_t n, int fmt, uint16_t *dst)
+__global__ void to_lp_kernel(const float *src, int64_t n, int fmt, void *out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const float x = src[i];
+    if (fmt == OM_FMT_F32) {
+        ((float *)out)[i] = x;
+        return;
+    }
+    uint16_t *dst = (uint16_t *)out;
     if (fmt == OM_FMT_BF16) {
         const uint32_t u = __float_as_uint(x);
         dst[i] = (x != x) ? (uint16_t)0x7FC0 : (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
@@ -347,14 +355,14 @@ using namespace om::lp;
 
 extern "C" int om_to_lp(const float *src, int64_t n, int32_t fmt, void *dst, void *stream)
 {
-    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16)
+    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16 && fmt != OM_FMT_F32)
         return fail(OM_EINVAL, "unknown storage format %d", fmt);
     if (n < 0)
         return fail(OM_ESHAPE, "negative size");
     if (n == 0)
         return OM_OK;
     hipLaunchKernelGGL(to_lp_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, as_stream(stream), src, n, fmt,
-                       (uint16_t *)dst);
+                       dst);
     return launch_check("to_lp_kernel");
 }
 
@@ -362,7 +370,7 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
 {
     if (!d || !ref16)
         return fail(OM_EINVAL, "om_sweep_lp: null argument");
-    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16)
+    if (fmt != OM_FMT_F16 && fmt != OM_FMT_BF16 && fmt != OM_FMT_F32)
         return fail(OM_EINVAL, "unknown storage format %d", fmt);
     if (d->n_src < 1 || d->n_src > 16)
         return fail(OM_ENSRC, "sweep needs 1..16 sources (got %d)", d->n_src);
@@ -382,7 +390,7 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
     SweepLPK P;
     memset(&P, 0, sizeof P);
     P.pinhole = 1;
-    P.ref = (const uint16_t *)ref16;
+    P.ref = ref16;
     P.mu = d->mu;
     P.sd = d->sd;
     P.ray = d->ray;
@@ -415,7 +423,7 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
         Q.k3 = (float)S.geo[15];
         if (S.geo[13] != 0.0 || S.geo[14] != 0.0 || S.geo[15] != 0.0)
             P.pinhole = 0;
-        Q.img = (const uint16_t *)S.img;
+        Q.img = S.img;
         Q.h = S.h;
         Q.w = S.w;
         for (int t = 0; t < 12; ++t)
@@ -440,10 +448,10 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
      {sweep_lp_kernel<2, FMT_, false>, sweep_lp_kernel<2, FMT_, true>},                                            \
      {sweep_lp_kernel<3, FMT_, false>, sweep_lp_kernel<3, FMT_, true>},                                            \
      {sweep_lp_kernel<4, FMT_, false>, sweep_lp_kernel<4, FMT_, true>}}
-    static const KFn table[2][4][2] = {OM_LP_ROW(OM_FMT_F16), OM_LP_ROW(OM_FMT_BF16)};
+    static const KFn table[3][4][2] = {OM_LP_ROW(OM_FMT_F16), OM_LP_ROW(OM_FMT_BF16), OM_LP_ROW(OM_FMT_F32)};
 #undef OM_LP_ROW
     hipStream_t st = as_stream(stream);
     ProfScope ps("sweep_lp", st);
-    hipLaunchKernelGGL(table[fmt == OM_FMT_BF16][k - 1][cd], dim3((unsigned)tiles), dim3(kThreads), lds, st, P);
+    hipLaunchKernelGGL(table[fmt - OM_FMT_F16][k - 1][cd], dim3((unsigned)tiles), dim3(kThreads), lds, st, P);
     return launch_check("sweep_lp_kernel");
 }

@@ -322,7 +322,7 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
               iters, dmin, dmax, seed, fmt: Optional[str] = None, stats: Optional[dict] = None) -> int:
     """PatchMatch repair of a depth map (_dense.pyx:566-666): only `active` pixels are re-estimated;
     depth / normal / score are updated in place.  Returns the number of accepted changes.
-    fmt "f16" / "bf16": the precision variant, with `srcs` as 16-bit patterns (to_lp).
+    fmt "f16" / "bf16" / "f32": the precision variant, with `srcs` in that storage (to_lp).
     stats: if given, receives "evaluated", the propagation cost evaluations actually computed."""
     S = len(srcs)
     if S == 0 or S > 16:
@@ -336,7 +336,7 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
         raise ValueError("Cr must have 3 values")
     if len(src_depths) != S:
         raise ValueError("src_depths must have one entry (or None) per source")
-    sdt = torch.float32 if fmt is None else torch.int16
+    sdt = torch.float32 if fmt is None or fmt == "f32" else torch.int16
     imgs = (ctypes.c_void_p * S)(*[_need(s, f"srcs[{k}]", sdt) for k, s in enumerate(srcs)])
     deps = (ctypes.c_void_p * S)(*[None if d is None else _need(d, f"src_depths[{k}]")
                                   for k, d in enumerate(src_depths)])
@@ -369,23 +369,28 @@ def pm_refine(ref, ray, Cr, srcs, src_depths, par, depth, normal, score, active,
 
 
 # ---------------------------------------------------------------- reduced precision (FP16 / BF16)
-FORMATS = {"f16": _lib.OM_FMT_F16, "bf16": _lib.OM_FMT_BF16}
+FORMATS = {"f16": _lib.OM_FMT_F16, "bf16": _lib.OM_FMT_BF16, "f32": _lib.OM_FMT_F32}
+
+
+def _lp_dtype(fmt: str):
+    return torch.float32 if fmt == "f32" else torch.int16
 
 
 def to_lp(a: torch.Tensor, fmt: str) -> torch.Tensor:
     """float32 -> FP16 / BF16 storage (round to nearest even), returned as an int16 tensor of
-    the same shape holding the 16-bit patterns."""
+    the same shape holding the 16-bit patterns; "f32": a float32 copy (the variants' FP32
+    arithmetic without the 16-bit rounding)."""
     f = FORMATS[fmt]
     src = a.contiguous()
     _need(src, "image")
-    out = torch.empty(src.shape, dtype=torch.int16, device=src.device)
+    out = torch.empty(src.shape, dtype=_lp_dtype(fmt), device=src.device)
     check(_lib.lib().om_to_lp(src.data_ptr(), src.numel(), f, out.data_ptr(), _stream()), "to_lp")
     return out
 
 
 def sweep_lp(ref16, mu, sd, srcs16, ray, geo, hyp: dict, r: int, top_k: int, fmt: str,
              state: Optional[SweepState] = None, i0: int = 0, nh: Optional[int] = None) -> SweepState:
-    """The precision variant of `sweep`: images as FP16/BF16 patterns (to_lp), FP32 warp, sums and
+    """The precision variant of `sweep`: images in FP16/BF16 (or FP32) storage (to_lp), FP32 warp, sums and
     NCC, FP64 plane depth and variance guard.  Same winner state; finish with sweep_finalize."""
     H, W = int(ref16.shape[0]), int(ref16.shape[1])
     mode, a, b, off, n_all = _hyp_inputs(hyp, H, W)
@@ -394,9 +399,9 @@ def sweep_lp(ref16, mu, sd, srcs16, ray, geo, hyp: dict, r: int, top_k: int, fmt
     if nh is None:
         nh = n_all - i0
     for t in [ref16] + list(srcs16):
-        _need(t, "16-bit image", torch.int16)
+        _need(t, "stored image", _lp_dtype(fmt))
     d, keep = _sweep_desc(mu, mu, sd, list(srcs16), ray, geo, r, top_k, state, mode, a, b, off, i0, nh,
-                          src_dtype=torch.int16)
+                          src_dtype=_lp_dtype(fmt))
     check(_lib.lib().om_sweep_lp(ctypes.byref(d), FORMATS[fmt], ref16.data_ptr(), _stream()), "sweep_lp")
     del keep
     return state

@@ -5,7 +5,8 @@
   batched = the per-view calls bit for bit, a HIP-graph replay = the eager run, and on a centre
   crop of one view of every size class the batched sweep and PatchMatch against the oracle.
 - Precision: a 4000x3000 reference with 12 sources, 384 planes + 8 PatchMatch iterations, images
-  in FP16 and in BF16 against the FP64 path (which is checked against the oracle on a crop).
+  in FP16, BF16 and FP32 storage against the FP64 path (which is checked against the oracle on a
+  crop), FP16 and BF16 also against FP32 storage.
 - Served: requests of the served size classes (1200x900, 2400x1800, 4000x3000), 4-12 sources and
   128-384 planes, served = the single-view calls bit for bit and alone = in the mix.
 """
@@ -145,12 +146,13 @@ def test_c3_crops_batched_sweep_and_patchmatch_match_oracle(c3):
 
 
 # ------------------------------------------------------------------------------ precision
-def test_c4_fp16_bf16_against_fp64_full_size():
+def test_c4_fp32_fp16_bf16_against_fp64_full_size():
     """4000x3000, 12 sources, 384 planes (11x11 window, top-2) + parabola + 8 PatchMatch iterations.
-    The FP64 path is checked against the oracle on the interior of a 160x160 centre crop; FP16 and
-    BF16 image storage are checked against the FP64 path: same winning plane on >= 98 % (FP16) /
-    85 % (BF16) of the valid pixels, |best score - FP64| at the 99th percentile <= 0.1 / 0.5, and
-    PatchMatch depth within 0.2 % of FP64 on >= 99 % of the pixels valid in both."""
+    The FP64 path is checked against the oracle on the interior of a 160x160 centre crop; FP32,
+    FP16 and BF16 image storage are checked against the FP64 path: same winning plane on >= 98 %
+    (FP16), 99.5 % (FP32) / 85 % (BF16) of the valid pixels, |best score - FP64| at the 99th
+    percentile <= 1e-3 / 0.1 / 0.5 (FP32 / FP16 / BF16), and PatchMatch depth within 0.2 % of FP64 on >= 99 % of the pixels valid in
+    both.  FP16 and BF16 are also reported against FP32 storage (their storage error alone)."""
     from py_ortho_amd import batch, block, dense, scene
     sc = scene.config_c4(device="cuda")
     ref = sc.ref.img.contiguous()
@@ -200,17 +202,27 @@ def test_c4_fp16_bf16_against_fp64_full_size():
                             np.ascontiguousarray(c(st64.idx)[inner]), "c4 fp64 crop")
     print("c4 fp64 vs oracle", stats)
     valid = (st64.best > 0.3) & (64.0 * sd > 0.5)
-    for fmt, win_min, db_max in (("f16", 0.98, 0.1), ("bf16", 0.85, 0.5)):
-        st, d, pm = run(fmt)
-        same = float((st.idx == st64.idx)[valid].float().mean())
-        db = (st.best - st64.best).abs()[valid].float()
+
+    def errors(a, b):
+        (st, d, pm), (st_b, d_b, pm_b) = a, b
+        same = float((st.idx == st_b.idx)[valid].float().mean())
+        db = (st.best - st_b.best).abs()[valid].float()
         db99 = float(torch.quantile(db[:: max(1, db.numel() // 1000000)], 0.99))
-        rel_sw = ((d - d64).abs() / d64)[valid].float()
-        both = torch.isfinite(pm) & torch.isfinite(pm64)
-        within = float((((pm - pm64).abs() / pm64)[both] < 2e-3).float().mean())
-        print(f"c4 {fmt}: same winner {same:.4f}, |d best| p99 {db99:.4g}, sweep depth rel p99 "
-              f"{float(torch.quantile(rel_sw[:: max(1, rel_sw.numel() // 1000000)], 0.99)):.3g}, "
-              f"PatchMatch within 0.2 % {within:.5f}")
+        rel_sw = ((d - d_b).abs() / d_b)[valid].float()
+        both = torch.isfinite(pm) & torch.isfinite(pm_b)
+        within = float((((pm - pm_b).abs() / pm_b)[both] < 2e-3).float().mean())
+        return same, db99, float(torch.quantile(rel_sw[:: max(1, rel_sw.numel() // 1000000)], 0.99)), within
+
+    run64 = (st64, d64, pm64)
+    run32 = run("f32")  # the variants' FP32 arithmetic with the images as they are
+    for fmt, win_min, db_max in (("f32", 0.995, 1e-3), ("f16", 0.98, 0.1), ("bf16", 0.85, 0.5)):
+        out = run32 if fmt == "f32" else run(fmt)
+        same, db99, rel99, within = errors(out, run64)
+        print(f"c4 {fmt} against FP64: same winner {same:.4f}, |d best| p99 {db99:.4g}, sweep depth rel p99 "
+              f"{rel99:.3g}, PatchMatch within 0.2 % {within:.5f}")
+        if fmt != "f32":
+            print(f"c4 {fmt} against FP32 storage: same winner %.4f, |d best| p99 %.4g, sweep depth rel p99 %.3g, "
+                  "PatchMatch within 0.2 %% %.5f" % errors(out, run32))
         assert same >= win_min and db99 <= db_max and within >= 0.99, fmt
 
 

@@ -1,6 +1,6 @@
-"""The precision variant of the sweep (FP16 / BF16 image storage, FP32 warp, sums and NCC): its
-error against the FP64 path on a rendered scene, stated as bounds on the best score, the winning
-plane and the depth; and the storage conversion against torch's own FP16 / BF16 rounding."""
+"""The precision variant of the sweep (FP16 / BF16 / FP32 image storage, FP32 warp, sums and NCC):
+its error against the FP64 path on a rendered scene, stated as bounds on the best score, the
+winning plane and the depth; and the storage conversion against torch's own FP16 / BF16 rounding."""
 import numpy as np
 import pytest
 import torch
@@ -21,12 +21,21 @@ def test_to_lp_matches_torch_rounding(fmt, dtype):
     assert torch.isnan(got.view(dtype)[~fin].float()).all()
 
 
+def test_to_lp_f32_is_a_copy():
+    from py_ortho_amd import dense
+    x = torch.randn(1001, device="cuda")
+    x[::97] = float("nan")
+    got = dense.to_lp(x, "f32")
+    assert got.dtype == torch.float32 and got.data_ptr() != x.data_ptr()
+    assert torch.equal(got.view(torch.int32), x.view(torch.int32))
+
+
 def _scene():
     from py_ortho_amd import scene
     return scene.config_c1(device="cuda", scale=0.75)
 
 
-@pytest.mark.parametrize("fmt,r", [("f16", 2), ("bf16", 2), ("f16", 5)])
+@pytest.mark.parametrize("fmt,r", [("f16", 2), ("bf16", 2), ("f16", 5), ("f32", 2)])
 def test_sweep_lp_error_against_fp64(fmt, r):
     from py_ortho_amd import dense, scene
     sc = _scene()
@@ -50,11 +59,12 @@ def test_sweep_lp_error_against_fp64(fmt, r):
     stats = (fmt, r, float(db.median()), float(torch.quantile(db, 0.99)), same, float(torch.quantile(rel, 0.99)))
     print("lp error", stats)
     # stated error bounds (measured on MI355X: f16 r=2 4.9e-3 / 0.050 / 0.991 / 3.7e-4, f16 r=5
-    # 1.2e-3 / 0.011 / 0.997 / 7.1e-5, bf16 r=2 0.038 / 0.32 / 0.893 / 6.6e-3): median and 99th
+    # 1.2e-3 / 0.011 / 0.997 / 7.1e-5, bf16 r=2 0.038 / 0.32 / 0.893 / 6.6e-3, f32 r=2 6.6e-6 / 6.5e-5 /
+    # 0.9997 / 5.6e-6): median and 99th
     # percentile of |best score - FP64|, share of identical winning planes, 99th percentile of the
     # relative depth difference, over pixels the reference would keep
     tol = {("f16", 2): (1e-2, 0.1, 0.98, 1e-3), ("f16", 5): (3e-3, 0.03, 0.99, 3e-4),
-           ("bf16", 2): (0.06, 0.5, 0.85, 2e-2)}[(fmt, r)]
+           ("bf16", 2): (0.06, 0.5, 0.85, 2e-2), ("f32", 2): (1e-4, 1e-3, 0.998, 1e-4)}[(fmt, r)]
     assert stats[2] < tol[0] and stats[3] < tol[1] and stats[4] > tol[2] and stats[5] < tol[3], stats
 
 
