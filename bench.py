@@ -43,7 +43,11 @@ FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
 HBM_PEAK = 8.0e12
 # HBM bytes of one full-block sweep_batch_kernel launch of the default workload (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply; profiles/block_c3.md)
-C3_SWEEP_TRAFFIC = 3.37e11
+C3_SWEEP_TRAFFIC = 3.405e11
+# share of the SIMD cycles in which that launch issues a VALU instruction (SQ_INSTS_VALU x 4 cycles over
+# 1024 SIMDs x 2.4 GHz x the launch; profiles/block_c3.md): the kernel is issue-bound, and its FLOP
+# fraction is low because a division or square root counts as one FLOP but takes ~11-18 instructions
+C3_SWEEP_VALU_ISSUE = 0.81
 
 
 def progress(msg: str) -> None:
@@ -454,6 +458,7 @@ def run_c3(args, torch):
         "roofline": {"bound": "valu", "engine": "VALU FP64 (no MFMA: gather-and-reduce)",
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": C3_SWEEP_TRAFFIC if args.scale == 1.0 else None,
+                     "valu_issue_frac": C3_SWEEP_VALU_ISSUE if args.scale == 1.0 else None,
                      "kernel": "sweep_batch_kernel",
                      "kernel_avg_ms": sw_avg * 1e3, "launches": sw_n, "flops_per_step": flops,
                      "pm_propagate_ms": pr_ms, "pm_random_ms": rn_ms},
