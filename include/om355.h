@@ -366,10 +366,11 @@ int om_argminmax0(const float *a, int32_t n, int64_t M, int32_t want_max, int64_
  * exposed so that tests can pin them against IEEE arithmetic.  Quotients sharing one reciprocal
  * of the divisor when the operands allow, plain division otherwise (meant for c > 1e-6): qa = a / c
  * and qb = b / c as the projection computes its pair, qs = (a + b) / c as the middle one of the
- * sweep NCC's three (b, a + b, a over c); fr = a - floor(a) as the bilinear tap's v_fract_f64.
- * Device arrays of n doubles. */
+ * sweep NCC's three (b, a + b, a over c); fr = a - floor(a) as the bilinear tap's v_fract_f64;
+ * sq = sqrt(c) as the sweep NCC takes it (its Newton sequence without operand scaling for
+ * 1e-6 < c < inf, sqrt otherwise).  Device arrays of n doubles. */
 int om_fp64_steps(const double *a, const double *b, const double *c, int64_t n, double *qa, double *qb, double *qs,
-                  double *fr, void *stream);
+                  double *fr, double *sq, void *stream);
 
 #ifdef __cplusplus
 }

@@ -109,7 +109,7 @@ SIGNATURES = {
     "om_topk_mean": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p, c_void_p]),
     "om_argminmax0": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
     "om_fp64_steps": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                c_void_p]),
+                                c_void_p, c_void_p]),
     "om_batch_create": (c_void_p, []),
     "om_batch_destroy": (c_int32, [c_void_p]),
     "om_batch_set_sweep": (c_int32, [c_void_p, ctypes.POINTER(OmSweepDesc), c_int32, ctypes.POINTER(c_void_p),

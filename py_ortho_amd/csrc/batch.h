@@ -21,7 +21,7 @@ struct om_batch {
     std::vector<void *> allocs;  // device buffers owned by the batch
 
     // plane sweep over all views: one persistent launch per tile height + one post-processing launch
-    int sw_n = 0, sw_k = 0, sw_ngroups = 0;
+    int sw_n = 0, sw_k = 0, sw_r = 0, sw_ngroups = 0;
     SweepGroup sw_groups[4];
     int64_t sw_px = 0;
     bool sw_merge = false, sw_final = false;

@@ -117,6 +117,22 @@ __device__ __forceinline__ void div2_shared(double xc, double yc, double zc, dou
     ny = q[1];
 }
 
+// IEEE square root of x for 2^-767 <= x < inf: the compiler's own f64 sqrt sequence (v_rsq_f64,
+// then Newton steps on the root g and the half reciprocal root h) without its operand scaling
+// (x < 2^-767) and its zero / infinity fix-up, which leave every x of that range unchanged.
+__device__ __forceinline__ double sqrt_normal(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 // Bilinear tap of a float32 image at 0 <= u <= w - 1.001, 0 <= v <= h - 1.001, in FP64 in the
 // order (a00 (1 - fx) + a01 fx) (1 - fy) + (a10 (1 - fx) + a11 fx) fy with x0 = (int)u and
 // fx = u - x0.  That difference is exact and equals u - floor(u), which v_fract_f64 returns; the

@@ -260,7 +260,7 @@ extern "C" int om_argminmax0(const float *a, int32_t n, int64_t M, int32_t want_
 
 namespace om {
 __global__ void fp64_steps_kernel(const double *a, const double *b, const double *c, int64_t n, double *qa, double *qb,
-                                  double *qs, double *fr)
+                                  double *qs, double *fr, double *sq)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -274,19 +274,22 @@ __global__ void fp64_steps_kernel(const double *a, const double *b, const double
     quotients_shared<3>(num, c[i], q);
     qs[i] = q[1];
     fr[i] = __builtin_amdgcn_fract(a[i]);
+    // the square root the sweep's NCC takes for 1e-6 < c < inf
+    const double s = c[i];
+    sq[i] = (s > 1e-6 && s < INFINITY) ? sqrt_normal(s) : sqrt(s);
 }
 }  // namespace om
 
 extern "C" int om_fp64_steps(const double *a, const double *b, const double *c, int64_t n, double *qa, double *qb,
-                             double *qs, double *fr, void *stream)
+                             double *qs, double *fr, double *sq, void *stream)
 {
     if (n < 0)
         return fail(OM_ESHAPE, "om_fp64_steps: negative size");
     if (n == 0)
         return OM_OK;
-    if (!a || !b || !c || !qa || !qb || !qs || !fr)
+    if (!a || !b || !c || !qa || !qb || !qs || !fr || !sq)
         return fail(OM_EINVAL, "om_fp64_steps: null pointer");
     hipLaunchKernelGGL(fp64_steps_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, as_stream(stream), a, b, c, n,
-                       qa, qb, qs, fr);
+                       qa, qb, qs, fr, sq);
     return launch_check("fp64_steps_kernel");
 }

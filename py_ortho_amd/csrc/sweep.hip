@@ -9,7 +9,7 @@
 //   W: warps the tile plus an r-pixel halo into LDS (FP64 projection + FP64 bilinear, the
 //      reference's exact operation sequence; NaN marks an invalid sample),
 //   H: forms the horizontal window sums of V, J, J^2, J*I for every halo row (FP64 running sums
-//      over 8-column runs, stored as float32 exactly where the reference rounds them),
+//      over 32-column runs, stored as float32 exactly where the reference rounds them),
 //   V: forms the vertical sums in FP64 and the NCC of each output pixel, and folds it into a
 //      per-pixel top-k kept in registers.
 // After the last source of a hypothesis the per-pixel winner state (best, prev, s_prev_best,
@@ -18,8 +18,8 @@
 // Exactness.  The library is compiled with -ffp-contract=off and uses IEEE division and square
 // root, so every sample, window sum and score is computed with the reference's operations in the
 // reference's order.  The reference forms window sums as running sums along the whole row
-// (_sweep.h:96-113) and down the whole column (_sweep.h:139-146); the kernel forms them over an
-// 8-sample run and a TH/4-row run.  Both equal the exact sum whenever no partial sum rounds
+// (_sweep.h:96-113) and down the whole column (_sweep.h:139-146); the kernel forms them over a
+// 32-column run and a TH/4-row run.  Both equal the exact sum whenever no partial sum rounds
 // (float32 inputs, FP64 accumulators: any window of values whose magnitudes span less than
 // 2^29), so the float32 outputs are bit-identical to the reference in practice and within an
 // FP64 rounding of it otherwise.
@@ -40,8 +40,6 @@ namespace om {
 
 constexpr int kTW = 64;                           // output columns per tile
 constexpr int kThreads = 256;                     // threads per workgroup
-constexpr int kRun = 8;                           // columns per thread in the horizontal pass
-constexpr int kRunsPerRow = kTW / kRun;           // 8
 constexpr int kVSeg = kThreads / kTW;             // 4 row segments per column in the vertical pass
 constexpr int kMaxR = 7;                          // largest window radius of the fused kernel
 // Tile heights: 4 * VR output rows, VR = output rows per thread in the vertical pass.  VR = 4
@@ -113,6 +111,12 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
 // (lanes spread over columns) hit few or no bank conflicts.
 constexpr int kTWP = kTW + 1;
 
+// The horizontal pass: one task per (halo row, run of kHRun columns), spread over all four waves
+// (the passes are separated by barriers, so a pass on fewer waves lengthens every workgroup's
+// step: 32-column runs on one wave issue a quarter fewer instructions yet made the 48-view block's
+// sweep 19 % slower).
+constexpr int kHRun = 8;
+constexpr int kHRuns = kTW / kHRun;
 __host__ __device__ constexpr int halo_stride(int r) { return (kTW + 2 * r) | 1; }
 
 // Constant planes are warped in groups of kPlaneGroup per pass over the halo: the plane-independent
@@ -125,35 +129,136 @@ constexpr int kPlaneGroup = OM_SWEEP_GROUP;
 
 __host__ __device__ constexpr int plane_group(bool constd) { return constd ? kPlaneGroup : 1; }
 
-// LDS bytes of one workgroup.  Constant planes: per halo pixel the reference and one warped source
-// value per plane of the group (the pixel index is recomputed from the halo position).  Otherwise:
-// per halo pixel the depth of the current plane, the pixel index, the reference and the warped
-// source.  Both: four planes of horizontal sums.
+// bytes of the valid-sample counts of the horizontal windows (one byte each, at most 2 r + 1)
+__host__ __device__ constexpr size_t count_bytes(int hr) { return ((size_t)hr * kTWP + 3) & ~(size_t)3; }
+
+// LDS bytes of one workgroup.  Per halo pixel its image pixel index, the reference and one warped
+// source value per plane of the group, plus (hypotheses that vary per pixel) the depth of the
+// current plane; three float planes of horizontal sums (J, J^2, J I) and their byte counts.  At
+// r = 5 and 16-row tiles that is 53 170 bytes with constant planes: three workgroups per CU.
 __host__ __device__ constexpr size_t sweep_lds(int th, int r, bool constd)
 {
-    return (size_t)(th + 2 * r) * halo_stride(r) * (constd ? 4 + 4 * plane_group(true) : 20) +
-           (size_t)4 * (th + 2 * r) * kTWP * 4;
+    return (size_t)(th + 2 * r) * halo_stride(r) * (constd ? 8 + 4 * plane_group(true) : 20) +
+           (size_t)3 * (th + 2 * r) * kTWP * 4 + count_bytes(th + 2 * r);
+}
+
+// Window radii with kernels of their own (the window loops unrolled, a horizontal run's samples
+// kept in registers): 2 (5x5, densify's default) and 5 (11x11); any other radius <= kMaxR runs the
+// generic kernels (R = -1, radius read from the descriptor).
+__host__ __device__ constexpr int radius_class(int r) { return r == 2 ? 0 : (r == 5 ? 1 : 2); }
+
+// Horizontal window sums of one run (_sweep.h:85-114): columns c0 .. c0 + kHRun - 1 of halo row
+// Jr / Ir, sums of V (count), J, J^2 and J I over 2 r + 1 columns, formed in FP64 and stored as
+// float exactly where the reference rounds them.  Each column adds the entering sample, then
+// subtracts the leaving one (the reference's order).  With a compile-time radius each sample's
+// validity and products are formed once and kept in registers until it leaves the window.
+#ifndef OM_SWEEP_HCACHE_MAXR
+#define OM_SWEEP_HCACHE_MAXR 7
+#endif
+
+template <int R>
+__device__ __forceinline__ void hsum_run(const float *Jr, const float *Ir, int r, float *h1, float *h2, float *h3,
+                                         uint8_t *hc)
+{
+    if constexpr (R >= 0 && R <= OM_SWEEP_HCACHE_MAXR) {
+        constexpr int NE = kHRun + 2 * R;
+        float ej[NE], ejj[NE], eji[NE];
+        int eok[NE];
+        auto elem = [&](int e) {
+            const float J = Jr[e];
+            const bool ok = J == J;
+            const float jv = ok ? J : 0.f;
+            ej[e] = jv;
+            ejj[e] = jv * jv;
+            eji[e] = jv * Ir[e];
+            eok[e] = ok;
+        };
+        int sv = 0;
+        double sj = 0.0, sjj = 0.0, sij = 0.0;
+#pragma unroll
+        for (int e = 0; e <= 2 * R; ++e) {
+            elem(e);
+            sv += eok[e];
+            sj = sj + (double)ej[e];
+            sjj = sjj + (double)ejj[e];
+            sij = sij + (double)eji[e];
+        }
+        hc[0] = (uint8_t)sv;
+        h1[0] = (float)sj;
+        h2[0] = (float)sjj;
+        h3[0] = (float)sij;
+#pragma unroll
+        for (int c = 1; c < kHRun; ++c) {
+            const int en = c + 2 * R, eo = c - 1;
+            elem(en);
+            sv += eok[en] - eok[eo];
+            sj = sj + (double)ej[en];
+            sjj = sjj + (double)ejj[en];
+            sij = sij + (double)eji[en];
+            sj = sj - (double)ej[eo];
+            sjj = sjj - (double)ejj[eo];
+            sij = sij - (double)eji[eo];
+            hc[c] = (uint8_t)sv;
+            h1[c] = (float)sj;
+            h2[c] = (float)sjj;
+            h3[c] = (float)sij;
+        }
+    } else {
+        int sv = 0;
+        double sj = 0.0, sjj = 0.0, sij = 0.0;
+        for (int t = 0; t <= 2 * r; ++t) {
+            const float J = Jr[t];
+            const bool ok = J == J;
+            const float jv = ok ? J : 0.f;
+            sv += ok;
+            sj = sj + (double)jv;
+            sjj = sjj + (double)(jv * jv);
+            sij = sij + (double)(jv * Ir[t]);
+        }
+        hc[0] = (uint8_t)sv;
+        h1[0] = (float)sj;
+        h2[0] = (float)sjj;
+        h3[0] = (float)sij;
+        for (int c = 1; c < kHRun; ++c) {
+            const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
+            const bool okn = Jn == Jn, oko = Jo == Jo;
+            const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
+            sv += (int)okn - (int)oko;
+            sj = sj + (double)jn;
+            sjj = sjj + (double)(jn * jn);
+            sij = sij + (double)(jn * Ir[c + 2 * r]);
+            sj = sj - (double)jo;
+            sjj = sjj - (double)(jo * jo);
+            sij = sij - (double)(jo * Ir[c - 1]);
+            hc[c] = (uint8_t)sv;
+            h1[c] = (float)sj;
+            h2[c] = (float)sjj;
+            h3[c] = (float)sij;
+        }
+    }
 }
 
 // One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
 // device memory (batched launches).  CD: the hypotheses are constant planes (OM_HYP_CONST), so the
 // depth of a plane is one value for the whole tile and needs no LDS; they are warped in groups of
-// G = kPlaneGroup.  HALF: the horizontal pass's half-wave task order (below).
-template <int K, int VR, bool CD, bool HALF, class D>
+// G = kPlaneGroup.  HALF: the horizontal pass's half-wave task order (below).  R: the window radius
+// when it has kernels of its own, else -1 (P.r).
+template <int K, int VR, bool CD, bool HALF, int R, class D>
 __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, double *lds_d)
 {
     constexpr int G = plane_group(CD);
-    const int r = P.r;
+    const int r = R >= 0 ? R : P.r;
     constexpr int TH = kVSeg * VR;
     const int HR = TH + 2 * r;          // halo rows
     const int HC = kTW + 2 * r;         // halo columns
     const int HCP = halo_stride(r);     // padded row stride
     const int NHP = HR * HCP;           // padded halo; the pad column is outside the image
     double *sD = lds_d;                                          // (HR, HCP) depth of the current hypothesis (!CD)
-    int *sP = CD ? nullptr : (int *)(sD + NHP);                  // (HR, HCP) pixel index, -1 outside (!CD)
-    float *sI = CD ? (float *)lds_d : (float *)(sP + NHP);       // (HR, HCP) reference
+    int *sP = CD ? (int *)lds_d : (int *)(sD + NHP);             // (HR, HCP) pixel index, -1 outside
+    float *sI = (float *)(sP + NHP);                             // (HR, HCP) reference
     float *sJ = sI + NHP;                                        // (G, HR, HCP) warped source, NaN = invalid
-    float *sH = sJ + G * NHP;                                    // (4, HR, kTWP) horizontal sums
+    float *sH = sJ + G * NHP;                                    // (3, HR, kTWP) horizontal sums of J, J^2, J I
+    uint8_t *sC = (uint8_t *)(sH + 3 * HR * kTWP);               // (HR, kTWP) valid samples of each window
 
     const int tid = threadIdx.x;
     const int tx0 = (tile % P.tiles_x) * kTW;
@@ -163,25 +268,18 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const int H = P.H, W = P.W;
     const size_t HW = (size_t)H * W;
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
-    // image pixel of the halo pixel at row hy, column hx, -1 outside the image (or in the pad column)
-    auto halo_at = [&](int hy, int hx) -> int {
-        const int gy = ty0 - r + hy, gx = tx0 - r + hx;
+    const int gx0 = tx0 - r, gy0 = ty0 - r;  // image position of halo pixel (0, 0)
+    // image pixel of the halo pixel hp, -1 outside the image (or in the pad column)
+    auto halo_px = [&](int hp) -> int {
+        const int hy = hp / HCP, hx = hp - hy * HCP;
+        const int gy = gy0 + hy, gx = gx0 + hx;
         return (hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W) ? gy * W + gx : -1;
     };
-    auto halo_px = [&](int hp) -> int {
-        const int hy = hp / HCP;
-        return halo_at(hy, hp - hy * HCP);
-    };
-    // a thread's halo pixels are tid, tid + kThreads, ...: its first one's row and column, and the
-    // step between them (the warp pass walks them without a division per pixel)
-    const int hy_t = tid / HCP, hx_t = tid - hy_t * HCP;
-    const int hy_step = kThreads / HCP, hx_step = kThreads - hy_step * HCP;
 
-    // ---- the tile + halo: pixel index (unless recomputed) and reference value
+    // ---- the tile + halo: pixel index and reference value
     for (int hp = tid; hp < NHP; hp += kThreads) {
         const int p = halo_px(hp);
-        if (!CD)
-            sP[hp] = p;
+        sP[hp] = p;
         sI[hp] = p >= 0 ? P.ref[p] : 0.f;
     }
 
@@ -264,8 +362,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             // works on its own.
             const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
                        *rayz = (const char *)(P.ray + 2 * HW);
-            int hyn = hy_t, hxn = hx_t;
-            int pn = tid < NHP ? (CD ? halo_at(hyn, hxn) : sP[tid]) : -1;
+            int pn = tid < NHP ? sP[tid] : -1;
             uint32_t on = (uint32_t)max(pn, 0) << 2;
             float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
                   rzn = *(const float *)(rayz + on);
@@ -276,13 +373,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     J[gg] = NAN;
                 const int p = pn;
                 const double rx = rxn, ry = ryn, rz = rzn;
-                hyn += hy_step;
-                hxn += hx_step;
-                if (hxn >= HCP) {
-                    hxn -= HCP;
-                    ++hyn;
-                }
-                pn = hp + kThreads < NHP ? (CD ? halo_at(hyn, hxn) : sP[hp + kThreads]) : -1;
+                pn = hp + kThreads < NHP ? sP[hp + kThreads] : -1;
                 on = (uint32_t)max(pn, 0) << 2;
                 rxn = *(const float *)(rayx + on);
                 ryn = *(const float *)(rayy + on);
@@ -332,66 +423,37 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 if (gg > 0)
                     __syncthreads();  // the previous plane's V pass has read sH
                 const float *sJg = sJ + gg * NHP;
-                // ---- H: horizontal window sums of one 8-column run per task (_sweep.h:85-114).
-                // A wave takes 8 rows x 8 runs.  HALF: each half-wave takes 8 rows x 4 runs (lane
-                // l: row (l / 4) % 8, run l % 4 + 4 (l / 32)), and with the odd row strides every
+                // ---- H: horizontal window sums of one 8-column run per task.  A wave takes 8
+                // rows x 8 runs.  HALF: each half-wave takes 8 rows x 4 runs (lane l: row
+                // (l / 4) % 8, run l % 4 + 4 (l / 32)), and with the odd row strides every
                 // half-wave's addresses fall in distinct LDS banks (SQ_LDS_BANK_CONFLICT of a
                 // 2000x1500, S = 8, 128-plane sweep: 0 against 1.1e9 cycles; the sweep 0.4-1.7 %
                 // faster).  The batched kernel keeps 4 rows x 8 runs per half-wave: its
                 // register-bound build is 1.1 % slower with the half-wave order.
-                const int ntask = HALF ? (HR + 7) / 8 * 64 : HR * kRunsPerRow;
+                const int ntask = HALF ? (HR + 7) / 8 * 64 : HR * kHRuns;
                 for (int t = tid; t < ntask; t += kThreads) {
                     int hr, c0;
                     if (HALF) {
                         const int l = t & 63;
                         hr = (t >> 6) * 8 + ((l >> 2) & 7);
-                        c0 = ((l & 3) + 4 * (l >> 5)) * kRun;
+                        c0 = ((l & 3) + 4 * (l >> 5)) * kHRun;
                         if (hr >= HR)
                             continue;
                     } else {
-                        hr = t / kRunsPerRow;
-                        c0 = (t % kRunsPerRow) * kRun;
+                        hr = t / kHRuns;
+                        c0 = (t % kHRuns) * kHRun;
                     }
-                    const float *Jr = sJg + hr * HCP, *Ir = sI + hr * HCP;
-                    int sv = 0;
-                    double sj = 0.0, sjj = 0.0, sij = 0.0;
-                    for (int t = 0; t <= 2 * r; ++t) {
-                        const float J = Jr[c0 + t];
-                        const bool ok = J == J;
-                        const float jv = ok ? J : 0.f;
-                        sv += ok;
-                        sj = sj + (double)jv;
-                        sjj = sjj + (double)(jv * jv);
-                        sij = sij + (double)(jv * Ir[c0 + t]);
-                    }
-                    float *h0 = sH + hr * kTWP, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
-                    h0[c0] = (float)sv;
-                    h1[c0] = (float)sj;
-                    h2[c0] = (float)sjj;
-                    h3[c0] = (float)sij;
-#pragma unroll
-                    for (int c = c0 + 1; c < c0 + kRun; ++c) {
-                        const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
-                        const bool okn = Jn == Jn, oko = Jo == Jo;
-                        const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
-                        sv += (int)okn - (int)oko;
-                        sj = sj + (double)jn;
-                        sjj = sjj + (double)(jn * jn);
-                        sij = sij + (double)(jn * Ir[c + 2 * r]);
-                        sj = sj - (double)jo;
-                        sjj = sjj - (double)(jo * jo);
-                        sij = sij - (double)(jo * Ir[c - 1]);
-                        h0[c] = (float)sv;
-                        h1[c] = (float)sj;
-                        h2[c] = (float)sjj;
-                        h3[c] = (float)sij;
-                    }
+                    const int o = hr * kTWP + c0;
+                    hsum_run<R>(sJg + hr * HCP + c0, sI + hr * HCP + c0, r, sH + o, sH + HR * kTWP + o,
+                                sH + 2 * HR * kTWP + o, sC + o);
                 }
                 __syncthreads();
                 // ---- V: vertical sums and NCC (_sweep.h:116-178)
                 {
-                    const float *h0 = sH + vc, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                    const float *h1 = sH + vc, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                    const uint8_t *h0 = sC + vc;
                     double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
                     for (int t = 0; t <= 2 * r; ++t) {
                         const int o = (vy0 + t) * kTWP;
                         c0 = c0 + (double)h0[o];
@@ -423,7 +485,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                             quotients_shared<3>(num, cov, mq);  // cov > 0.9: IEEE quotients
                             const double mj = mq[0], m2 = mq[1], mij = mq[2];
                             double sj = m2 - mj * mj;
-                            sj = sj > 1e-6 ? sqrt(sj) : 1e-3;
+                            sj = sj > 1e-6 ? (sj < INFINITY ? sqrt_normal(sj) : sqrt(sj)) : 1e-3;
                             const double q = (mij - (double)mu_[j] * mj) / ((double)sd_[j] * sj);
                             out = q > 1.0 ? 1.0f : (q < -1.0 ? -1.0f : (float)q);
                         }
@@ -486,19 +548,26 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     }
 }
 
-template <int K, int VR, bool CD>
-__global__ void __launch_bounds__(kThreads) sweep_kernel(const SweepK P)
+// Waves per SIMD the fused sweep kernels are compiled for: three 256-thread workgroups share a CU
+// (their LDS fits three times into 160 KB), which needs at most 168 VGPRs per lane.
+#ifndef OM_SWEEP_WAVES
+#define OM_SWEEP_WAVES 3
+#endif
+#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(OM_SWEEP_WAVES, OM_SWEEP_WAVES)))
+
+template <int K, int VR, bool CD, int R>
+__global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_kernel(const SweepK P)
 {
     extern __shared__ double lds_d[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
-    sweep_tile<K, VR, CD, true>(P, lb / P.chunks, lb % P.chunks, lds_d);
+    sweep_tile<K, VR, CD, true, R>(P, lb / P.chunks, lb % P.chunks, lds_d);
 }
 
 // Persistent batch: work items (view, tile, chunk) of the views vidx[0..nv) from one queue; item0[j]
 // is the first item of view vidx[j].  Every wave of a workgroup reads the same item index from LDS,
 // so all of them leave the loop together once the queue is empty.
-template <int K, int VR, bool CD>
-__global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *views, const int *vidx, const int *item0,
+template <int K, int VR, bool CD, int R>
+__global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_batch_kernel(const SweepK *views, const int *vidx, const int *item0,
                                                                int nv, int total, int *counter)
 {
     extern __shared__ double lds_d[];
@@ -521,7 +590,7 @@ __global__ void __launch_bounds__(kThreads) sweep_batch_kernel(const SweepK *vie
         }
         const OM_CONST_AS SweepK &P = as_const(views)[as_const(vidx)[lo]];
         const int local = it - as_const(item0)[lo];
-        sweep_tile<K, VR, CD, false>(P, local / P.chunks, local % P.chunks, lds_d);
+        sweep_tile<K, VR, CD, false, R>(P, local / P.chunks, local % P.chunks, lds_d);
     }
 }
 
@@ -726,6 +795,61 @@ static int build_params(const om_sweep_desc *d, SweepK &P, bool fused = true)
     return OM_OK;
 }
 
+// kernel instances: top-k 1..4, tile height 8 or 16, constant planes or not, radius class
+using SingleFn = void (*)(const SweepK);
+using BatchFn = void (*)(const SweepK *, const int *, const int *, int, int, int *);
+
+template <int K, int VR, bool CD>
+static SingleFn single_r(int rc)
+{
+    return rc == 0 ? sweep_kernel<K, VR, CD, 2> : (rc == 1 ? sweep_kernel<K, VR, CD, 5> : sweep_kernel<K, VR, CD, -1>);
+}
+
+template <int K, int VR, bool CD>
+static BatchFn batch_r(int rc)
+{
+    return rc == 0 ? sweep_batch_kernel<K, VR, CD, 2>
+                   : (rc == 1 ? sweep_batch_kernel<K, VR, CD, 5> : sweep_batch_kernel<K, VR, CD, -1>);
+}
+
+template <int K>
+static SingleFn single_k(int th, bool cd, int rc)
+{
+    if (th == 16)
+        return cd ? single_r<K, 4, true>(rc) : single_r<K, 4, false>(rc);
+    return cd ? single_r<K, 2, true>(rc) : single_r<K, 2, false>(rc);
+}
+
+template <int K>
+static BatchFn batch_k(int th, bool cd, int rc)
+{
+    if (th == 16)
+        return cd ? batch_r<K, 4, true>(rc) : batch_r<K, 4, false>(rc);
+    return cd ? batch_r<K, 2, true>(rc) : batch_r<K, 2, false>(rc);
+}
+
+static SingleFn single_kernel(int k, int th, bool cd, int r)
+{
+    const int rc = radius_class(r);
+    switch (k) {
+    case 1: return single_k<1>(th, cd, rc);
+    case 2: return single_k<2>(th, cd, rc);
+    case 3: return single_k<3>(th, cd, rc);
+    default: return single_k<4>(th, cd, rc);
+    }
+}
+
+static BatchFn batch_kernel(int k, int th, bool cd, int r)
+{
+    const int rc = radius_class(r);
+    switch (k) {
+    case 1: return batch_k<1>(th, cd, rc);
+    case 2: return batch_k<2>(th, cd, rc);
+    case 3: return batch_k<3>(th, cd, rc);
+    default: return batch_k<4>(th, cd, rc);
+    }
+}
+
 static int choose_chunks(const om_sweep_desc *d, const SweepK &P)
 {
     const int64_t tiles = (int64_t)P.tiles_x * P.tiles_y;
@@ -754,13 +878,7 @@ static int sweep_launch(SweepK &Pl, int chunks, float *part, hipStream_t st, boo
     const size_t lds = sweep_lds(Pl.TH, Pl.r, cd);
     const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
     ProfScope ps(prof ? "sweep" : nullptr, st);
-    using KFn = void (*)(const SweepK);
-    static const KFn table[4][2][2] = {
-        {{sweep_kernel<1, 2, false>, sweep_kernel<1, 2, true>}, {sweep_kernel<1, 4, false>, sweep_kernel<1, 4, true>}},
-        {{sweep_kernel<2, 2, false>, sweep_kernel<2, 2, true>}, {sweep_kernel<2, 4, false>, sweep_kernel<2, 4, true>}},
-        {{sweep_kernel<3, 2, false>, sweep_kernel<3, 2, true>}, {sweep_kernel<3, 4, false>, sweep_kernel<3, 4, true>}},
-        {{sweep_kernel<4, 2, false>, sweep_kernel<4, 2, true>}, {sweep_kernel<4, 4, false>, sweep_kernel<4, 4, true>}}};
-    hipLaunchKernelGGL(table[Pl.k - 1][Pl.TH == 16][cd], grid, dim3(kThreads), lds, st, Pl);
+    hipLaunchKernelGGL(single_kernel(Pl.k, Pl.TH, cd, Pl.r), grid, dim3(kThreads), lds, st, Pl);
     if ((rc = launch_check("sweep_kernel")))
         return rc;
     if (Pl.chunks > 1) {
@@ -927,22 +1045,6 @@ extern "C" int om_sweep_finalize(const om_sweep_desc *d, float *depth, void *str
 
 namespace om {
 
-using BatchFn = void (*)(const SweepK *, const int *, const int *, int, int, int *);
-
-static BatchFn batch_kernel(int k, int th, bool cd)
-{
-    static const BatchFn table[4][2][2] = {
-        {{sweep_batch_kernel<1, 2, false>, sweep_batch_kernel<1, 2, true>},
-         {sweep_batch_kernel<1, 4, false>, sweep_batch_kernel<1, 4, true>}},
-        {{sweep_batch_kernel<2, 2, false>, sweep_batch_kernel<2, 2, true>},
-         {sweep_batch_kernel<2, 4, false>, sweep_batch_kernel<2, 4, true>}},
-        {{sweep_batch_kernel<3, 2, false>, sweep_batch_kernel<3, 2, true>},
-         {sweep_batch_kernel<3, 4, false>, sweep_batch_kernel<3, 4, true>}},
-        {{sweep_batch_kernel<4, 2, false>, sweep_batch_kernel<4, 2, true>},
-         {sweep_batch_kernel<4, 4, false>, sweep_batch_kernel<4, 4, true>}}};
-    return table[k - 1][th == 16][cd];
-}
-
 // persistent launches of a batch: one per (tile height, constant planes) pair
 static int plan_groups(const SweepK *P, int nviews, int k, int r, int cus, int grid_blocks, std::vector<int> &vidx,
                        std::vector<int> &item0, SweepGroup *groups, int &ng, bool skip_empty)
@@ -971,7 +1073,7 @@ static int plan_groups(const SweepK *P, int nviews, int k, int r, int cus, int g
             g.total = (int)total;
             g.lds = sweep_lds(th, r, cd);
             int per_cu = 1;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th, cd), kThreads,
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th, cd, r), kThreads,
                                                              g.lds) != hipSuccess ||
                 per_cu < 1)
                 per_cu = 1;
@@ -1057,6 +1159,7 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
         return rc;
     b->sw_n = nviews;
     b->sw_k = k;
+    b->sw_r = r;
     b->sw_px = px0[nviews];
     b->sw_merge = merge;
     b->sw_final = final_;
@@ -1074,7 +1177,7 @@ int batch_sweep_enqueue(om_batch *b, hipStream_t st)
     for (int gi = 0; gi < b->sw_ngroups; ++gi) {
         const SweepGroup &g = b->sw_groups[gi];
         ProfScope pk("sweep_batch_kernel", st);
-        hipLaunchKernelGGL(batch_kernel(b->sw_k, g.th, g.cd), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st,
+        hipLaunchKernelGGL(batch_kernel(b->sw_k, g.th, g.cd, b->sw_r), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st,
                            (const SweepK *)b->sw_views, b->sw_vidx + g.first, b->sw_item0 + g.first, g.nv, g.total,
                            b->sw_counter + gi);
     }
@@ -1195,7 +1298,7 @@ extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float 
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counter, 4);
     for (int gi = 0; gi < ng; ++gi) {
         const SweepGroup &g = groups[gi];
-        hipLaunchKernelGGL(batch_kernel(k, g.th, g.cd), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st, dviews,
+        hipLaunchKernelGGL(batch_kernel(k, g.th, g.cd, r), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st, dviews,
                            (const int *)(dtab + L.vidx) + g.first, (const int *)(dtab + L.item0) + g.first, g.nv,
                            g.total, counter + gi);
     }

@@ -1,4 +1,4 @@
-"""The FP64 kernels' two arithmetic shortcuts against IEEE double arithmetic, bit for bit.
+"""The FP64 kernels' arithmetic shortcuts against IEEE double arithmetic, bit for bit.
 
 - Quotients over one divisor share its reciprocal when the operands lie in the range where the
   compiler's own division would neither scale its operands nor fix up its result, and divide
@@ -7,6 +7,9 @@
   rounded, so it is the exact expectation.
 - The bilinear tap's fraction u - (int)u, computed as v_fract_f64 (fr), for u >= 0 (the tap only
   runs inside [0, w - 1.001]).
+- The sweep NCC's square root of the source variance (sq): for 1e-6 < v < inf the compiler's
+  Newton sequence without its operand scaling, sqrt otherwise.  NumPy's sqrt is IEEE correctly
+  rounded.
 
 Operands are drawn inside the shared-reciprocal range, at its edges and far beyond it, with zeros,
 signed zeros, infinities and NaNs mixed in; the divisor always satisfies the projection's own
@@ -27,11 +30,12 @@ N = 1 << 20
 
 def _run(a, b, c):
     A, B, C = (torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda() for v in (a, b, c))
-    qa, qb, qs, fr = (torch.empty_like(A) for _ in range(4))
+    qa, qb, qs, fr, sq = (torch.empty_like(A) for _ in range(5))
     check(_lib.lib().om_fp64_steps(A.data_ptr(), B.data_ptr(), C.data_ptr(), A.numel(), qa.data_ptr(),
-                                   qb.data_ptr(), qs.data_ptr(), fr.data_ptr(), _stream()), "om_fp64_steps")
+                                   qb.data_ptr(), qs.data_ptr(), fr.data_ptr(), sq.data_ptr(), _stream()),
+          "om_fp64_steps")
     torch.cuda.synchronize()
-    return qa.cpu().numpy(), qb.cpu().numpy(), qs.cpu().numpy(), fr.cpu().numpy()
+    return qa.cpu().numpy(), qb.cpu().numpy(), qs.cpu().numpy(), fr.cpu().numpy(), sq.cpu().numpy()
 
 
 def _same(x, y):
@@ -67,7 +71,7 @@ def _operands(kind, rng):
 def test_projection_quotients_are_ieee_division(kind):
     rng = np.random.default_rng(["typical", "lower_edge", "upper_edge", "any"].index(kind) + 11)
     a, b, c = _operands(kind, rng)
-    qa, qb, qs, _ = _run(a, b, c)
+    qa, qb, qs, _, _ = _run(a, b, c)
     with np.errstate(all="ignore"):
         ea, eb, es = a / c, b / c, (a + b) / c
     bad_a, bad_b, bad_s = ~_same(qa, ea), ~_same(qb, eb), ~_same(qs, es)
@@ -85,7 +89,31 @@ def test_tap_fraction_is_u_minus_its_integer_part():
         np.arange(N // 4, dtype=np.float64) / 4.0,         # exact quarters, whole numbers among them
     ])
     u[::1009] = np.nextafter(np.floor(u[::1009]) + 1.0, 0.0)  # just below the next integer
-    _, _, _, fr = _run(u, u, np.ones_like(u))
+    _, _, _, fr, _ = _run(u, u, np.ones_like(u))
     exp = u - np.trunc(u)
     bad = ~_same(fr, exp)
     assert not bad.any(), (u[bad][:3], fr[bad][:3], exp[bad][:3])
+
+
+def test_variance_square_root_is_ieee_sqrt():
+    rng = np.random.default_rng(17)
+    q = N // 8
+    v = np.concatenate([
+        rng.uniform(0, 1, q),                              # variances of [0, 1] images
+        np.exp2(rng.uniform(-20, 20, q)),                  # 1e-6 .. 1e6
+        1e-6 * (1 + np.exp2(rng.uniform(-52, 0, q))),      # just above the guard
+        np.exp2(rng.uniform(-1074, 1024, q)),              # anything positive a double holds
+        np.exp2(rng.uniform(1000, 1024, q)),               # up to the largest doubles
+        rng.integers(1, 1 << 40, q).astype(np.float64) ** 2,  # exact squares
+        np.nextafter(rng.integers(1, 1 << 20, q).astype(np.float64) ** 2, np.inf),  # an ulp off a square
+        rng.uniform(-1, 1e-6, q),                          # at or below the guard (plain sqrt)
+    ])
+    v[::4099] = np.inf
+    v[1::4093] = 0.0
+    v[2::4091] = np.nan
+    v[3::4089] = np.finfo(np.float64).max
+    _, _, _, _, sq = _run(v, v, v)
+    with np.errstate(all="ignore"):
+        exp = np.sqrt(v)
+    bad = ~_same(sq, exp)
+    assert not bad.any(), (v[bad][:3], sq[bad][:3], exp[bad][:3])
