@@ -138,14 +138,32 @@ __device__ __forceinline__ double sqrt_normal(double x)
 // fx = u - x0.  That difference is exact and equals u - floor(u), which v_fract_f64 returns; the
 // two rows are addressed by 32-bit byte offsets from the image base (callers check that images
 // hold fewer than 2^30 pixels).
-__device__ __forceinline__ double bil_f64(const float *im, int w, double u, double v)
+__device__ __forceinline__ void bil_load(const float *im, int w, double u, double v, float &a00, float &a01, float &a10,
+                                         float &a11, double &fx, double &fy)
 {
     const uint32_t x0 = (uint32_t)(int)u, y0 = (uint32_t)(int)v;
-    const double fx = __builtin_amdgcn_fract(u), fy = __builtin_amdgcn_fract(v);
+    fx = __builtin_amdgcn_fract(u);
+    fy = __builtin_amdgcn_fract(v);
     const uint32_t o0 = (__umul24(y0, (uint32_t)w) + x0) << 2, o1 = o0 + ((uint32_t)w << 2);
     const float *r0 = (const float *)((const char *)im + o0), *r1 = (const float *)((const char *)im + o1);
-    const double a00 = r0[0], a01 = r0[1], a10 = r1[0], a11 = r1[1];
+    a00 = r0[0];
+    a01 = r0[1];
+    a10 = r1[0];
+    a11 = r1[1];
+}
+
+// the bilinear's arithmetic on taps loaded by bil_load
+__device__ __forceinline__ double bil_mix(double a00, double a01, double a10, double a11, double fx, double fy)
+{
     return (a00 * (1.0 - fx) + a01 * fx) * (1.0 - fy) + (a10 * (1.0 - fx) + a11 * fx) * fy;
+}
+
+__device__ __forceinline__ double bil_f64(const float *im, int w, double u, double v)
+{
+    float a00, a01, a10, a11;
+    double fx, fy;
+    bil_load(im, w, u, v, a00, a01, a10, a11, fx, fy);
+    return bil_mix(a00, a01, a10, a11, fx, fy);
 }
 
 // most pixels an image read by bil_f64 may hold (32-bit byte offsets)

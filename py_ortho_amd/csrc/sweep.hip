@@ -363,6 +363,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
                        *rayz = (const char *)(P.ray + 2 * HW);
             int pn = tid < NHP ? sP[tid] : -1;
+            int pn2 = tid + kThreads < NHP ? sP[tid + kThreads] : -1;
             uint32_t on = (uint32_t)max(pn, 0) << 2;
             float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
                   rzn = *(const float *)(rayz + on);
@@ -373,7 +374,8 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     J[gg] = NAN;
                 const int p = pn;
                 const double rx = rxn, ry = ryn, rz = rzn;
-                pn = hp + kThreads < NHP ? sP[hp + kThreads] : -1;
+                pn = pn2;
+                pn2 = hp + 2 * kThreads < NHP ? sP[hp + 2 * kThreads] : -1;
                 on = (uint32_t)max(pn, 0) << 2;
                 rxn = *(const float *)(rayx + on);
                 ryn = *(const float *)(rayy + on);
