@@ -25,7 +25,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import batch, dense
+from . import batch, dense, formats, fusion
 
 __all__ = ["BlockOptions", "BlockDepth"]
 
@@ -40,6 +40,16 @@ class BlockOptions:
     pm_step: int = 2
     pm_geo_weight: float = 0.5
     geo_tol: float = 0.01
+    # PatchMatch's pixels and tolerance.  "all": every pixel with geo_tol (the 48-view block
+    # workload, all of it one HIP graph).  "inconsistent": the reference's pass-2 inputs built from
+    # the sweep's maps (densify.py:611-634): the pixels that are valid in the image and inconsistent
+    # with fewer than min_views - 1 neighbours, below min_ncc or without a depth, and the median of
+    # the neighbours' relative tolerances (densify.py:574-582, with the view's median sweep depth in
+    # place of its tie points' median depth, which a block of images alone does not have).
+    repair: str = "all"
+    min_ncc: float = 0.3
+    min_views: int = 0          # 0: 3 when the median view has >= 3 neighbours, else 2 (densify.py:560)
+    px_tol: float = 1.5         # reprojection tolerance of the consistency check, pixels
 
 
 def _geo18(cam_s, cam_r) -> np.ndarray:
@@ -92,6 +102,7 @@ class BlockDepth:
         self.active = [torch.ones(im.shape, dtype=torch.uint8, device=dev) for im in imgs]
         self.states = [dense.SweepState(int(im.shape[0]), int(im.shape[1]), dev) for im in imgs]
         self._batch: Optional[batch.Batch] = None
+        self._pm = False
 
     # ---- accounting
     def pixels(self) -> List[int]:
@@ -127,11 +138,48 @@ class BlockDepth:
     def seed(k: int) -> int:
         return (k * 2654435761) % 2 ** 32  # the reference's per-view seed (densify.py:634)
 
+    # ---- the reference's repair inputs (BlockOptions.repair == "inconsistent")
+    def min_views(self) -> int:
+        o = self.opt
+        return o.min_views or (3 if np.median([len(nb) for nb in self.nbrs]) >= 3 else 2)
+
+    def rel_tols(self, k: int, dmed: float) -> Dict[int, float]:
+        """Relative depth tolerance per neighbour of view k (densify.py:574-582): the depth change
+        that moves a point by px_tol pixels between the two views, in [0.01, 0.05]."""
+        C = [np.asarray(c.C, np.float64) for c in self.cams]
+        f = float(self.cams[k].f)
+        return {j: float(np.clip(self.opt.px_tol * dmed / max(f * float(np.linalg.norm(C[k] - C[j])), 1e-9),
+                                 0.01, 0.05)) for j in self.nbrs[k]}
+
+    def repair_inputs(self):
+        """Per view (active, geo_tol, score) from the sweep's maps (self.raw, self.states): the
+        consistency count against the neighbours' maps (om_consistency), the score rounded to
+        float16 and widened as densify stores it, active = finite(image) & (count < min_views - 1 |
+        score < min_ncc | ~finite(depth)) (densify.py:619) and geo_tol = median(rel_tols)
+        (densify.py:632)."""
+        o, mv = self.opt, self.min_views()
+        dm = {k: self.raw[k] for k in range(self.n)}
+        out = []
+        for k in range(self.n):
+            D = self.raw[k]
+            fin = D[torch.isfinite(D)]
+            dmed = float(fin.median()) if fin.numel() else 1.0
+            tol = self.rel_tols(k, dmed)
+            cnt = fusion.consistency(k, D, self.nbrs[k], dm, self.cams, o.px_tol, tol)
+            S = formats.to_f32(formats.to_f16(self.states[k].best))
+            active = torch.isfinite(self.imgs[k]) & ((cnt < mv - 1) | (S < o.min_ncc) | ~torch.isfinite(D))
+            out.append((active.to(torch.uint8).contiguous(), float(np.median(list(tol.values()))), S))
+        return out
+
     # ---- batched run
     def plan(self, tune: Optional[Dict[Tuple[int, int], Tuple[int, int]]] = None, pm: bool = True) -> batch.Batch:
-        """Plans the batch; tune maps a size class (H, W) to (tile_h, chunks) (autotune.tune_block)."""
+        """Plans the batch; tune maps a size class (H, W) to (tile_h, chunks) (autotune.tune_block).
+        With repair == "inconsistent" the batch holds the sweep only: run() then builds the repair
+        inputs from its maps and runs PatchMatch as a second batch."""
         if self._batch is not None:
             self._batch.close()
+        if self.opt.repair not in ("all", "inconsistent"):
+            raise ValueError(f"BlockOptions.repair must be 'all' or 'inconsistent' (got {self.opt.repair!r})")
         o = self.opt
         B = batch.Batch()
         sv = []
@@ -139,7 +187,8 @@ class BlockDepth:
             th, ch = (tune or {}).get(self.size_class(k), (0, 0))
             sv.append(self.sweep_view(k, th, ch))
         B.set_sweep(sv, o.window, o.top_k)
-        if pm and o.pm_iters > 0:
+        self._pm = pm and o.pm_iters > 0
+        if self._pm and o.repair == "all":
             B.set_pm([self.pm_view(k) for k in range(self.n)], o.pm_window, o.pm_step, o.top_k, o.pm_geo_weight,
                      o.pm_iters, init_normals=True)
         self._batch = B
@@ -149,8 +198,31 @@ class BlockDepth:
         if self._batch is None:
             self.plan()
         self._batch.run()
+        if self._pm and self.opt.repair == "inconsistent":
+            self._run_repair()
+
+    def _run_repair(self):
+        """PatchMatch of every view over the reference's repair inputs, as one batch."""
+        o = self.opt
+        inputs = self.repair_inputs()
+        views = []
+        for k, (active, geo_tol, S) in enumerate(inputs):
+            self.active[k].copy_(active)
+            self.score[k].copy_(S)
+            v = self.pm_view(k)
+            v.geo_tol, v.score_from = geo_tol, None
+            views.append(v)
+        B = batch.Batch()
+        try:
+            B.set_pm(views, o.pm_window, o.pm_step, o.top_k, o.pm_geo_weight, o.pm_iters, init_normals=True)
+            B.run()
+        finally:
+            torch.cuda.current_stream().synchronize()
+            B.close()
 
     def capture(self):
+        if self.opt.repair != "all":
+            raise ValueError("capture() records the 'all' repair mode; 'inconsistent' builds its inputs on the host")
         if self._batch is None:
             self.plan()
         self._batch.capture()
@@ -185,11 +257,18 @@ class BlockDepth:
                                                    st))
         if o.pm_iters <= 0:
             return
+        inputs = self.repair_inputs() if o.repair == "inconsistent" else None
         for k in range(self.n):
             P = self.pm_view(k)
             self.depth[k].copy_(self.raw[k])
-            self.score[k].copy_(self.states[k].best)
+            geo_tol = o.geo_tol
+            if inputs is None:
+                self.score[k].copy_(self.states[k].best)
+            else:
+                active, geo_tol, S = inputs[k]
+                self.active[k].copy_(active)
+                self.score[k].copy_(S)
             self.normal[k].copy_(batch.init_normals(self.raw[k], self.rays[k]))
             dense.pm_refine(P.ref, P.ray, P.Cr, P.srcs, P.src_depths, P.par, self.depth[k], self.normal[k],
                             self.score[k], self.active[k], o.pm_window, o.pm_step, o.top_k, o.pm_geo_weight,
-                            o.geo_tol, o.pm_iters, P.dmin, P.dmax, P.seed)
+                            geo_tol, o.pm_iters, P.dmin, P.dmax, P.seed)

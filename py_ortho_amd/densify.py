@@ -9,9 +9,10 @@
    refinement levels); the score is kept as float16 and widened again, as the reference stores it
    (densify.py:590-592, :613);
 3. pass 2: per view the geometric consistency count against the neighbours' pass-1 maps
-   (`om_consistency`), the PatchMatch inputs -- the pixels to repair, normals, the neighbours' maps
-   as source depths, the median relative tolerance, the depth range or the map's own span, the
-   view's seed -- and `pm_refine`, then the final filter (densify.py:609-641);
+   (`om_consistency`) and the PatchMatch inputs -- the pixels to repair, normals, the neighbours'
+   maps as source depths, the median relative tolerance, the depth range or the map's own span, the
+   view's seed; then PatchMatch of all views in one batch (`om_batch`; each view reads only pass-1
+   maps, so the views are independent) and the final filter (densify.py:609-641);
 4. fusion: the reference's sequential fusion over the final maps (`om_fuse`, densify.py:656-666).
 
 Nothing goes through files: the `.npy` depth-map set is written only when `opt.keep_depthmaps`
@@ -19,7 +20,8 @@ is set, with the reference's names and bytes.  Image decode stays on the host: `
 scale)` and `load_mask(path, w, h)` are the reference's imageio.load_rgb and masks.load_mask unless
 other loaders are passed.
 
-Views are processed one after another on the device stream, each launch spanning the whole GPU.
+Pass 1 runs the views one after another on the device stream, each launch spanning the whole GPU;
+pass 2's PatchMatch runs all views of a block together.
 """
 from __future__ import annotations
 
@@ -32,7 +34,7 @@ from typing import Callable, Dict, Optional
 import numpy as np
 import torch
 
-from . import dense, formats, fusion, mvs, plan, prep
+from . import batch, dense, formats, fusion, mvs, plan, prep
 
 log = logging.getLogger(__name__)
 
@@ -137,46 +139,96 @@ class DensifyRun:
             self.low[k] = low
 
     # ------------------------------------------------------------------ pass 2 (densify.py:609-641)
-    def repair(self, k: int) -> torch.Tensor:
-        """View k's final depth map: PatchMatch over its inconsistent pixels (when opt.patchmatch),
-        then NaN where the score is below min_ncc or the window is flat."""
+    def repair_inputs(self, k: int):
+        """View k's repair inputs (densify.py:611-634): (D, S, pm) with D a copy of the pass-1 map and
+        S the widened float16 score, which PatchMatch refines in place, and pm the remaining
+        pm_refine arguments as a batch.PMView, or None when there is nothing to repair (PatchMatch
+        off, no active pixel or no neighbour map)."""
         opt, rec = self.opt, self.rec
         D = self.raw[k].clone()
         S = formats.to_f32(self.score16[k])
-        if opt.patchmatch:
-            tol = self.rel_tols(k)
-            cnt = fusion.consistency(k, D, self.nbrs[k], self.raw, self.cams, self.px_tol, tol)
-            grey, _, cr = self.view(k)
-            active = (torch.isfinite(grey) & ((cnt < self.min_views - 1) | (S < opt.min_ncc) | ~torch.isfinite(D)))
-            active = active.to(torch.uint8).contiguous()
-            src_ids = [j for j in self.nbrs[k] if j in self.raw]
-            n_active = int(active.sum())
-            if n_active and src_ids:
-                ray = mvs.rays_world(tuple(D.shape), cr, self.device)
-                normal = mvs.init_normals(D, ray)
-                srcs = [self.view(j)[0] for j in src_ids]
-                par = np.array([np.r_[self.cams[j].R.ravel(), self.cams[j].C, self.cams[j].f, self.cams[j].k1,
-                                      self.cams[j].k2, self.cams[j].k3, self.cams[j].cx, self.cams[j].cy]
-                                for j in src_ids], np.float64)
-                sdeps = [self.raw[j] for j in src_ids]
-                rng = plan.depth_range(rec, k)
-                if rng is None:                      # the map's own span (np.nanmin / np.nanmax)
-                    fin = D[torch.isfinite(D)]
-                    rng = (float(fin.min()), float(fin.max())) if fin.numel() else (float("nan"), float("nan"))
-                seed = k * 2654435761 % 2 ** 32
-                ch = dense.pm_refine(grey, ray, np.asarray(cr.C, np.float64), srcs, sdeps, par, D, normal, S, active,
-                                     opt.pm_window, opt.pm_step, opt.top_k, opt.pm_geo_weight,
-                                     float(np.median(list(tol.values()))), opt.pm_iters, rng[0], rng[1], seed)
-                self.stats["repaired"] += int(ch)
-                self.stats["active"] += n_active
-            self.stats["pixels"] += int(torch.isfinite(D).sum())
+        if not opt.patchmatch:
+            return D, S, None
+        tol = self.rel_tols(k)
+        cnt = fusion.consistency(k, D, self.nbrs[k], self.raw, self.cams, self.px_tol, tol)
+        grey, _, cr = self.view(k)
+        # active = finite(grey) & (cnt < min_views - 1 | S < min_ncc | ~finite(D))  (densify.py:619)
+        active = (torch.isfinite(grey) & ((cnt < self.min_views - 1) | (S < opt.min_ncc) | ~torch.isfinite(D)))
+        active = active.to(torch.uint8).contiguous()
+        src_ids = [j for j in self.nbrs[k] if j in self.raw]
+        self.stats["pixels"] += int(torch.isfinite(D).sum())
+        n_active = int(active.sum())
+        if not n_active or not src_ids:
+            return D, S, None
+        self.stats["active"] += n_active
+        ray = mvs.rays_world(tuple(D.shape), cr, self.device)
+        normal = mvs.init_normals(D, ray)
+        par = np.array([np.r_[self.cams[j].R.ravel(), self.cams[j].C, self.cams[j].f, self.cams[j].k1,
+                              self.cams[j].k2, self.cams[j].k3, self.cams[j].cx, self.cams[j].cy]
+                        for j in src_ids], np.float64)
+        rng = plan.depth_range(rec, k)
+        if rng is None:                      # the map's own span (np.nanmin / np.nanmax)
+            fin = D[torch.isfinite(D)]
+            rng = (float(fin.min()), float(fin.max())) if fin.numel() else (float("nan"), float("nan"))
+        pm = batch.PMView(grey, ray, np.asarray(cr.C, np.float64), [self.view(j)[0] for j in src_ids],
+                          [self.raw[j] for j in src_ids], par, D, normal, S, active,
+                          float(np.median(list(tol.values()))), rng[0], rng[1], k * 2654435761 % 2 ** 32)
+        return D, S, pm
+
+    def _final(self, k: int, D: torch.Tensor, S: torch.Tensor) -> torch.Tensor:
         # D[(S < min_ncc) | low] = NaN with the float32 score (densify.py:638)
-        bad = (S < opt.min_ncc) | self.low[k]
+        bad = (S < self.opt.min_ncc) | self.low[k]
         return torch.where(bad, torch.full_like(D, float("nan")), D).contiguous()
 
-    def pass2(self):
-        for k in sorted(self.raw):
-            self.final[k] = self.repair(k)
+    def repair(self, k: int) -> torch.Tensor:
+        """View k's final depth map: PatchMatch over its inconsistent pixels (when opt.patchmatch),
+        then NaN where the score is below min_ncc or the window is flat."""
+        opt = self.opt
+        D, S, pm = self.repair_inputs(k)
+        if pm is not None:
+            ch = dense.pm_refine(pm.ref, pm.ray, pm.Cr, pm.srcs, pm.src_depths, pm.par, pm.depth, pm.normal, pm.score,
+                                 pm.active, opt.pm_window, opt.pm_step, opt.top_k, opt.pm_geo_weight, pm.geo_tol,
+                                 opt.pm_iters, pm.dmin, pm.dmax, pm.seed)
+            self.stats["repaired"] += int(ch)
+        return self._final(k, D, S)
+
+    def pass2(self, batched: bool = True, max_batch_pixels: int = 64 << 20):
+        """Pass 2 of every view.  Each view's repair reads only pass-1 maps (its own and its
+        neighbours'), so the views are independent: batched, their PatchMatch runs as one om_batch
+        (all views' colour passes together) per group of up to max_batch_pixels reference pixels,
+        with the same inputs and results as the per-view calls."""
+        keys = sorted(self.raw)
+        if not batched or not self.opt.patchmatch:
+            for k in keys:
+                self.final[k] = self.repair(k)
+            return
+        opt = self.opt
+        group, px = [], 0
+
+        def flush():
+            nonlocal group, px
+            todo = [(k, D, S, pm) for k, D, S, pm in group if pm is not None]
+            if todo:
+                B = batch.Batch()
+                try:
+                    B.set_pm([pm for _, _, _, pm in todo], opt.pm_window, opt.pm_step, opt.top_k, opt.pm_geo_weight,
+                             opt.pm_iters)
+                    B.run()
+                    self.stats["repaired"] += sum(B.pm_changed())
+                finally:
+                    B.close()
+            for k, D, S, _ in group:
+                self.final[k] = self._final(k, D, S)
+            group, px = [], 0
+
+        for k in keys:
+            D, S, pm = self.repair_inputs(k)
+            n = int(D.numel()) if pm is not None else 0
+            if group and px + n > max_batch_pixels:
+                flush()
+            group.append((k, D, S, pm))
+            px += n
+        flush()
 
     # ------------------------------------------------------------------ fusion (densify.py:652-666)
     def fuse(self) -> formats.DenseCloud:

@@ -10,12 +10,12 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def _block(scale=0.1, planes=24, iters=2):
+def _block(scale=0.1, planes=24, iters=2, **kw):
     from py_ortho_amd import block, scene
     sizes = [(int(round(w * scale)), int(round(h * scale))) for w, h in scene.C3_SIZES]
     cams, counts, ext = scene.survey_cameras(4, 4, 3, 90.0, sizes, 0.9375, 0.7, 0.6, (2, 6))
     sv = scene.make_survey(4, cams, counts, ext, 90.0, device="cuda")
-    opt = block.BlockOptions(num_depths=planes, pm_iters=iters)
+    opt = block.BlockOptions(num_depths=planes, pm_iters=iters, **kw)
     blk = block.BlockDepth([v.img for v in sv.views], [v.cam for v in sv.views], sv.nbrs,
                            [(sv.dmin, sv.dmax)] * len(sv.views), opt)
     return sv, blk
@@ -75,4 +75,45 @@ def test_block_depth_accuracy():
         rel = torch.cat([((d - v.depth).abs() / v.depth).flatten() for d, v in zip(maps, sv.views)])
         rel = rel[torch.isfinite(rel)]
         assert float(rel.median()) < 0.01
+    blk.close()
+
+
+def test_block_reference_repair_batch_equals_single_view():
+    """repair="inconsistent": PatchMatch over the reference's pass-2 inputs (densify.py:611-634)
+    built from the sweep's maps.  The batched run equals the per-view calls bit for bit; the inputs
+    are the reference's expressions over the consistency count (om_consistency, pinned to the
+    reference's _consistency by golden vectors) and the float16-rounded score."""
+    from py_ortho_amd import formats, fusion
+    sv, blk = _block(repair="inconsistent", min_ncc=0.5)
+    blk.plan()
+    blk.run()
+    torch.cuda.synchronize()
+    raw_b, dep_b, nrm_b, sc_b, act_b = (_bits(blk.raw), _bits(blk.depth), _bits(blk.normal), _bits(blk.score),
+                                        [a.cpu().numpy().copy() for a in blk.active])
+    inputs = blk.repair_inputs()
+    mv = blk.min_views()
+    n_act = n_px = 0
+    for k, (active, geo_tol, S) in enumerate(inputs):
+        # the inputs, recomputed here from their parts
+        D = blk.raw[k]
+        fin = D[torch.isfinite(D)]
+        tol = blk.rel_tols(k, float(fin.median()))
+        assert set(tol) == set(blk.nbrs[k]) and all(0.01 <= t <= 0.05 for t in tol.values())
+        assert geo_tol == float(np.median(list(tol.values())))
+        cnt = fusion.consistency(k, D, blk.nbrs[k], {j: blk.raw[j] for j in range(blk.n)}, blk.cams, blk.opt.px_tol,
+                                 tol).cpu().numpy()
+        s16 = formats.to_f16(blk.states[k].best).cpu().numpy().astype(np.float32)
+        d, img = D.cpu().numpy(), blk.imgs[k].cpu().numpy()
+        want = np.isfinite(img) & ((cnt < mv - 1) | (s16 < blk.opt.min_ncc) | ~np.isfinite(d))
+        assert np.array_equal(active.cpu().numpy().astype(bool), want), k
+        assert np.array_equal(act_b[k].astype(bool), want), k
+        n_act += int(want.sum())
+        n_px += want.size
+    assert 0 < n_act < n_px, (n_act, n_px)       # some pixels repaired, others kept
+    blk.run_single()
+    torch.cuda.synchronize()
+    for got, want in ((raw_b, blk.raw), (dep_b, blk.depth), (nrm_b, blk.normal), (sc_b, blk.score)):
+        assert all(np.array_equal(a, b) for a, b in zip(got, _bits(want)))
+    with pytest.raises(ValueError):
+        blk.capture()
     blk.close()

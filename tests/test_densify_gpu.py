@@ -107,3 +107,27 @@ def test_densify_exact_matches_reference_bit_for_bit(name, tmp_path):
         assert _same(np.load(wd / f"{stem}.npy"), z[f"{name}_{stem}"]), stem
     for field in ("xyz", "rgb", "views", "cam"):
         assert _same(getattr(cloud, field), z[f"{name}_{field}"]), field
+
+
+@pytest.mark.parametrize("max_batch_pixels", [64 << 20, 20000])
+def test_densify_pass2_batched_equals_per_view(max_batch_pixels):
+    """Pass 2 with every view's PatchMatch in one om_batch (or, with a small pixel budget, in several
+    batches) gives the per-view pm_refine calls' final maps and change counts bit for bit."""
+    from py_ortho_amd import densify, mvs
+    z = np.load(GOLDEN)
+    ar, rec, gains, biases, load_rgb, load_mask = _inputs(z)
+    opt = mvs.DepthOptions(**SETTINGS["pm"])
+    run = densify.DensifyRun(ar, rec, gains, biases, opt, load_rgb, load_mask)
+    run.pass1()
+    raw = {k: v.clone() for k, v in run.raw.items()}
+    run.pass2(batched=False)
+    per_view, stats = {k: v.clone() for k, v in run.final.items()}, dict(run.stats)
+    run.stats = {"repaired": 0, "active": 0, "pixels": 0}
+    run.final = {}
+    run.pass2(batched=True, max_batch_pixels=max_batch_pixels)
+    torch.cuda.synchronize()
+    assert sorted(run.final) == sorted(per_view)
+    for k in per_view:
+        assert torch.equal(run.final[k].view(torch.int32), per_view[k].view(torch.int32)), k
+        assert torch.equal(run.raw[k].view(torch.int32), raw[k].view(torch.int32)), k   # pass-1 maps untouched
+    assert run.stats == stats and stats["active"] > 0 and stats["repaired"] > 0, (run.stats, stats)
