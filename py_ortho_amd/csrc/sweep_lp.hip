@@ -15,7 +15,8 @@
 // best scores differ by more than that).
 //
 // LDS rows are padded to an odd number of words so that the horizontal pass (lanes spread over
-// rows) and the vertical pass (lanes spread over columns) read distinct banks.
+// rows) and the vertical pass (lanes spread over columns) read distinct banks.  Constant planes are
+// warped two at a time, and the window loops of radii 2 and 5 are compiled with the radius known.
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -97,12 +98,113 @@ __device__ __forceinline__ void topk_ins(float (&t)[K], float v)
     }
 }
 
-// CD: constant planes (OM_HYP_CONST): one depth per plane, no per-pixel depth in LDS
-template <int K, int FMT, bool CD>
+// Window radii with kernels of their own (loops unrolled, a horizontal run's samples kept in
+// registers): 2 and 5; other radii <= kMaxR run the generic kernel (R = -1, radius from P.r).
+__host__ __device__ constexpr int lp_radius_class(int r) { return r == 2 ? 0 : (r == 5 ? 1 : 2); }
+
+// Constant planes are warped two at a time: the plane-independent term fmaf(g0, rx, fmaf(g1, ry,
+// g2 rz)) of each coordinate is formed once per halo pixel and source for both.
+constexpr int kPlaneGroupLP = 2;
+
+__host__ __device__ constexpr int lp_group(bool cd) { return cd ? kPlaneGroupLP : 1; }
+
+// bytes of the valid-sample counts of the horizontal windows (one byte each, at most 2 r + 1)
+__host__ __device__ constexpr size_t lp_count_bytes(int hr) { return ((size_t)hr * (kTW + 1) + 3) & ~(size_t)3; }
+
+// LDS bytes: per halo pixel its index, the reference, one warped value per plane of the group and
+// (per-pixel hypotheses) the plane's depth; three float planes of horizontal sums and their counts
+__host__ __device__ constexpr size_t lp_lds(int r, bool cd)
+{
+    return (size_t)(kTH + 2 * r) * ((kTW + 2 * r) | 1) * 4 * (2 + lp_group(cd) + (cd ? 0 : 1)) +
+           (size_t)3 * (kTH + 2 * r) * (kTW + 1) * 4 + lp_count_bytes(kTH + 2 * r);
+}
+
+// FP32 horizontal running sums of one 8-column run (counts as bytes); with a compile-time radius
+// each sample's value and its reference are read once and kept until it leaves the window
+template <int R>
+__device__ __forceinline__ void lp_hsum_run(const float *Jr, const float *Ir, int r, float *h1, float *h2, float *h3,
+                                            uint8_t *hc)
+{
+    if constexpr (R >= 0) {
+        constexpr int NE = kRun + 2 * R;
+        float ej[NE], ei[NE];
+        int eok[NE];
+        auto elem = [&](int e) {
+            const float J = Jr[e];
+            const bool ok = J == J;
+            ej[e] = ok ? J : 0.f;
+            ei[e] = Ir[e];
+            eok[e] = ok;
+        };
+        int sv = 0;
+        float sj = 0.f, sjj = 0.f, sij = 0.f;
+#pragma unroll
+        for (int a = 0; a <= 2 * R; ++a) {
+            elem(a);
+            sv += eok[a];
+            sj += ej[a];
+            sjj = fmaf(ej[a], ej[a], sjj);
+            sij = fmaf(ej[a], ei[a], sij);
+        }
+        hc[0] = (uint8_t)sv;
+        h1[0] = sj;
+        h2[0] = sjj;
+        h3[0] = sij;
+#pragma unroll
+        for (int c = 1; c < kRun; ++c) {
+            const int en = c + 2 * R, eo = c - 1;
+            elem(en);
+            const float jn = ej[en], jo = ej[eo];
+            sv += eok[en] - eok[eo];
+            sj += jn - jo;
+            sjj = fmaf(jn, jn, fmaf(-jo, jo, sjj));
+            sij = fmaf(jn, ei[en], fmaf(-jo, ei[eo], sij));
+            hc[c] = (uint8_t)sv;
+            h1[c] = sj;
+            h2[c] = sjj;
+            h3[c] = sij;
+        }
+    } else {
+        int sv = 0;
+        float sj = 0.f, sjj = 0.f, sij = 0.f;
+        for (int a = 0; a <= 2 * r; ++a) {
+            const float J = Jr[a];
+            const bool ok = J == J;
+            const float jv = ok ? J : 0.f;
+            sv += ok;
+            sj += jv;
+            sjj = fmaf(jv, jv, sjj);
+            sij = fmaf(jv, Ir[a], sij);
+        }
+        hc[0] = (uint8_t)sv;
+        h1[0] = sj;
+        h2[0] = sjj;
+        h3[0] = sij;
+#pragma unroll
+        for (int c = 1; c < kRun; ++c) {
+            const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
+            const bool okn = Jn == Jn, oko = Jo == Jo;
+            const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
+            sv += (int)okn - (int)oko;
+            sj += jn - jo;
+            sjj = fmaf(jn, jn, fmaf(-jo, jo, sjj));
+            sij = fmaf(jn, Ir[c + 2 * r], fmaf(-jo, Ir[c - 1], sij));
+            hc[c] = (uint8_t)sv;
+            h1[c] = sj;
+            h2[c] = sjj;
+            h3[c] = sij;
+        }
+    }
+}
+
+// CD: constant planes (OM_HYP_CONST): one depth per plane, warped in groups of kPlaneGroupLP.
+// R: the window radius when it has kernels of its own, else -1.
+template <int K, int FMT, bool CD, int R>
 __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
 {
     extern __shared__ float lds_f[];
-    const int r = P.r;
+    constexpr int G = lp_group(CD);
+    const int r = R >= 0 ? R : P.r;
     const int HR = kTH + 2 * r, HC = kTW + 2 * r;
     const int HCP = HC | 1;          // padded halo row stride (odd)
     constexpr int kTWP = kTW + 1;    // padded row stride of the horizontal sums
@@ -110,8 +212,9 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     float *sD = lds_f;                             // (HR, HCP) depth of the current plane (!CD)
     int *sP = (int *)(CD ? lds_f : sD + NH);      // (HR, HCP) pixel index, -1 outside
     float *sI = (float *)(sP + NH);  // (HR, HCP) reference
-    float *sJ = sI + NH;             // (HR, HCP) warped source, NaN = invalid
-    float *sH = sJ + NH;             // (4, HR, kTWP) horizontal sums
+    float *sJ = sI + NH;             // (G, HR, HCP) warped source, NaN = invalid
+    float *sH = sJ + G * NH;         // (3, HR, kTWP) horizontal sums of J, J^2, J I
+    uint8_t *sC = (uint8_t *)(sH + 3 * HR * kTWP);  // (HR, kTWP) valid samples of each window
 
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int tx0 = (tile % P.tiles_x) * kTW, ty0 = (tile / P.tiles_x) * kTH;
@@ -134,7 +237,7 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     const int ox = tx0 + vc;
     float mu_[kVR], sd_[kVR], st_best[kVR], st_prev[kVR], st_spb[kVR], st_snb[kVR];
     int st_idx[kVR];
-    float tk[kVR][K];
+    float tk[G][kVR][K];
 #pragma unroll
     for (int j = 0; j < kVR; ++j) {
         const int oy = ty0 + vy0 + j;
@@ -151,10 +254,17 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     const float win = (float)((2 * r + 1) * (2 * r + 1));
     __syncthreads();
 
-    for (int i = P.i0; i < P.i0 + P.nh; ++i) {
-        float dc = 0.f;
+    for (int i = P.i0; i < P.i0 + P.nh; i += G) {
+        const int ng = min(G, P.i0 + P.nh - i);   // planes of this group (wave-uniform)
+        float dc[G];
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg)
+            dc[gg] = 0.f;
         if (CD) {
-            dc = (float)(1.0 / (double)P.ha[i - P.i0]);  // plane parameter in FP64, rounded once
+#pragma unroll
+            for (int gg = 0; gg < G; ++gg)
+                if (gg < ng)
+                    dc[gg] = (float)(1.0 / (double)P.ha[i + gg - P.i0]);  // plane parameter in FP64, rounded once
         } else {
             for (int o = tid; o < NH; o += kThreads) {
                 const int p = sP[o];
@@ -167,18 +277,21 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
             }
         }
 #pragma unroll
-        for (int j = 0; j < kVR; ++j)
+        for (int gg = 0; gg < G; ++gg)
 #pragma unroll
-            for (int a = 0; a < K; ++a)
-                tk[j][a] = -INFINITY;
+            for (int j = 0; j < kVR; ++j)
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+                    tk[gg][j][a] = -INFINITY;
         if (!CD)
             __syncthreads();
 
         for (int s = 0; s < P.n; ++s) {
             const SrcLP &S = P.src[s];
             const float *g = S.g;
-            // ---- W: warp the halo (FP32 projection, FP32 bilinear of 16-bit taps); each step loads
-            // the ray of its next halo pixel (pixel 0's outside the image) before it works on its own
+            // ---- W: warp the halo (FP32 projection, FP32 bilinear of 16-bit taps) for every plane
+            // of the group; each step loads the ray of its next halo pixel (pixel 0's outside the
+            // image) before it works on its own
             const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
                        *rayz = (const char *)(P.ray + 2 * HW);
             int pn = tid < NH ? sP[tid] : -1;
@@ -193,125 +306,126 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                 rxn = *(const float *)(rayx + on);
                 ryn = *(const float *)(rayy + on);
                 rzn = *(const float *)(rayz + on);
-                float J = NAN;
+                float J[G];
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg)
+                    J[gg] = NAN;
                 if (p >= 0) {
-                    const float d = CD ? dc : sD[o];
-                    const float X = fmaf(d, fmaf(g[0], rx, fmaf(g[1], ry, g[2] * rz)), g[9]);
-                    const float Y = fmaf(d, fmaf(g[3], rx, fmaf(g[4], ry, g[5] * rz)), g[10]);
-                    const float Z = fmaf(d, fmaf(g[6], rx, fmaf(g[7], ry, g[8] * rz)), g[11]);
-                    if (Z > 1e-6f) {
-                        const float iz = __builtin_amdgcn_rcpf(Z);
-                        const float nx = X * iz, ny = Y * iz;
-                        float dd = S.f;   // radial model of _sweep.h:55-56 in FP32
-                        if (!P.pinhole) {
-                            const float r2 = fmaf(nx, nx, ny * ny);
-                            dd = S.f * fmaf(r2, fmaf(r2, fmaf(r2, S.k3, S.k2), S.k1), 1.0f);
-                        }
-                        const float u = fmaf(dd, nx, S.cx), v = fmaf(dd, ny, S.cy);
-                        if (u >= 0.f && v >= 0.f && u <= S.wl && v <= S.hl) {
-                            const int x0 = (int)u, y0 = (int)v;
-                            const float fx = u - (float)x0, fy = v - (float)y0;
-                            const size_t o = (size_t)y0 * S.w + x0;
-                            const float a00 = ld_img<FMT>(S.img, o), a01 = ld_img<FMT>(S.img, o + 1);
-                            const float a10 = ld_img<FMT>(S.img, o + S.w), a11 = ld_img<FMT>(S.img, o + S.w + 1);
-                            const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
-                            const float val = fmaf(fy, t1 - t0, t0);
-                            if (isfinite(val))
-                                J = val;
+                    const float ax = fmaf(g[0], rx, fmaf(g[1], ry, g[2] * rz));
+                    const float ay = fmaf(g[3], rx, fmaf(g[4], ry, g[5] * rz));
+                    const float az = fmaf(g[6], rx, fmaf(g[7], ry, g[8] * rz));
+#pragma unroll
+                    for (int gg = 0; gg < G; ++gg) {
+                        if (gg >= ng)
+                            break;
+                        const float d = CD ? dc[gg] : sD[o];
+                        const float X = fmaf(d, ax, g[9]);
+                        const float Y = fmaf(d, ay, g[10]);
+                        const float Z = fmaf(d, az, g[11]);
+                        if (Z > 1e-6f) {
+                            const float iz = __builtin_amdgcn_rcpf(Z);
+                            const float nx = X * iz, ny = Y * iz;
+                            float dd = S.f;   // radial model of _sweep.h:55-56 in FP32
+                            if (!P.pinhole) {
+                                const float r2 = fmaf(nx, nx, ny * ny);
+                                dd = S.f * fmaf(r2, fmaf(r2, fmaf(r2, S.k3, S.k2), S.k1), 1.0f);
+                            }
+                            const float u = fmaf(dd, nx, S.cx), v = fmaf(dd, ny, S.cy);
+                            if (u >= 0.f && v >= 0.f && u <= S.wl && v <= S.hl) {
+                                const int x0 = (int)u, y0 = (int)v;
+                                const float fx = u - (float)x0, fy = v - (float)y0;
+                                const size_t t = (size_t)y0 * S.w + x0;
+                                const float a00 = ld_img<FMT>(S.img, t), a01 = ld_img<FMT>(S.img, t + 1);
+                                const float a10 = ld_img<FMT>(S.img, t + S.w), a11 = ld_img<FMT>(S.img, t + S.w + 1);
+                                const float t0 = fmaf(fx, a01 - a00, a00), t1 = fmaf(fx, a11 - a10, a10);
+                                const float val = fmaf(fy, t1 - t0, t0);
+                                if (isfinite(val))
+                                    J[gg] = val;
+                            }
                         }
                     }
-                }
-                sJ[o] = J;
-            }
-            __syncthreads();
-            // ---- H: FP32 running sums over 8-column runs; a wave's lanes take 8 rows x 8 runs.
-            // (The FP64 kernel's half-wave order, free of bank conflicts, measured 2-4 % slower
-            // here.)
-            for (int t = tid; t < HR * (kTW / kRun); t += kThreads) {
-                const int hr = t / (kTW / kRun), c0 = (t % (kTW / kRun)) * kRun;
-                const float *Jr = sJ + hr * HCP, *Ir = sI + hr * HCP;
-                int sv = 0;
-                float sj = 0.f, sjj = 0.f, sij = 0.f;
-                for (int a = 0; a <= 2 * r; ++a) {
-                    const float J = Jr[c0 + a];
-                    const bool ok = J == J;
-                    const float jv = ok ? J : 0.f;
-                    sv += ok;
-                    sj += jv;
-                    sjj = fmaf(jv, jv, sjj);
-                    sij = fmaf(jv, Ir[c0 + a], sij);
-                }
-                float *h0 = sH + hr * kTWP, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
-                h0[c0] = (float)sv;
-                h1[c0] = sj;
-                h2[c0] = sjj;
-                h3[c0] = sij;
-#pragma unroll
-                for (int c = c0 + 1; c < c0 + kRun; ++c) {
-                    const float Jn = Jr[c + 2 * r], Jo = Jr[c - 1];
-                    const bool okn = Jn == Jn, oko = Jo == Jo;
-                    const float jn = okn ? Jn : 0.f, jo = oko ? Jo : 0.f;
-                    sv += (int)okn - (int)oko;
-                    sj += jn - jo;
-                    sjj = fmaf(jn, jn, fmaf(-jo, jo, sjj));
-                    sij = fmaf(jn, Ir[c + 2 * r], fmaf(-jo, Ir[c - 1], sij));
-                    h0[c] = (float)sv;
-                    h1[c] = sj;
-                    h2[c] = sjj;
-                    h3[c] = sij;
-                }
-            }
-            __syncthreads();
-            // ---- V: FP32 vertical sums, NCC (variance and its guard in FP64)
-            {
-                const float *h0 = sH + vc, *h1 = h0 + HR * kTWP, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
-                float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-                for (int a = 0; a <= 2 * r; ++a) {
-                    const int o = (vy0 + a) * kTWP;
-                    c0 += h0[o];
-                    c1 += h1[o];
-                    c2 += h2[o];
-                    c3 += h3[o];
                 }
 #pragma unroll
-                for (int j = 0; j < kVR; ++j) {
-                    if (j > 0) {
-                        const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
-                        c0 += h0[on] - h0[oo];
-                        c1 += h1[on] - h1[oo];
-                        c2 += h2[on] - h2[oo];
-                        c3 += h3[on] - h3[oo];
+                for (int gg = 0; gg < G; ++gg)
+                    sJ[gg * NH + o] = J[gg];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int gg = 0; gg < G; ++gg) {
+                if (gg >= ng)
+                    break;
+                if (gg > 0)
+                    __syncthreads();  // the previous plane's V pass has read sH
+                const float *sJg = sJ + gg * NH;
+                // ---- H: FP32 running sums over 8-column runs; a wave's lanes take 8 rows x 8 runs.
+                // (The FP64 kernel's half-wave order, free of bank conflicts, measured 2-4 % slower
+                // here.)
+                for (int t = tid; t < HR * (kTW / kRun); t += kThreads) {
+                    const int hr = t / (kTW / kRun), c0 = (t % (kTW / kRun)) * kRun;
+                    const int oh = hr * kTWP + c0;
+                    lp_hsum_run<R>(sJg + hr * HCP + c0, sI + hr * HCP + c0, r, sH + oh, sH + HR * kTWP + oh,
+                                   sH + 2 * HR * kTWP + oh, sC + oh);
+                }
+                __syncthreads();
+                // ---- V: FP32 vertical sums, NCC (variance and its guard in FP64)
+                {
+                    const float *h1 = sH + vc, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
+                    const uint8_t *h0 = sC + vc;
+                    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+#pragma unroll
+                    for (int a = 0; a <= 2 * r; ++a) {
+                        const int o = (vy0 + a) * kTWP;
+                        c0 += (float)h0[o];
+                        c1 += h1[o];
+                        c2 += h2[o];
+                        c3 += h3[o];
                     }
-                    float out = -1.0f;
-                    if (c0 > 0.9f * win) {  // >= 90 % of the window valid
-                        const float ic = __builtin_amdgcn_rcpf(c0);
-                        const float mj = c1 * ic, m2 = c2 * ic, mij = c3 * ic;
-                        const double vj = (double)m2 - (double)mj * (double)mj;
-                        const float sj = vj > 1e-6 ? (float)sqrt(vj) : 1e-3f;
-                        const float q = (mij - mu_[j] * mj) * __builtin_amdgcn_rcpf(sd_[j] * sj);
-                        out = q > 1.0f ? 1.0f : (q < -1.0f ? -1.0f : q);
+#pragma unroll
+                    for (int j = 0; j < kVR; ++j) {
+                        if (j > 0) {
+                            const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
+                            c0 += (float)h0[on] - (float)h0[oo];
+                            c1 += h1[on] - h1[oo];
+                            c2 += h2[on] - h2[oo];
+                            c3 += h3[on] - h3[oo];
+                        }
+                        float out = -1.0f;
+                        if (c0 > 0.9f * win) {  // >= 90 % of the window valid
+                            const float ic = __builtin_amdgcn_rcpf(c0);
+                            const float mj = c1 * ic, m2 = c2 * ic, mij = c3 * ic;
+                            const double vj = (double)m2 - (double)mj * (double)mj;
+                            const float sj = vj > 1e-6 ? (float)sqrt(vj) : 1e-3f;
+                            const float q = (mij - mu_[j] * mj) * __builtin_amdgcn_rcpf(sd_[j] * sj);
+                            out = q > 1.0f ? 1.0f : (q < -1.0f ? -1.0f : q);
+                        }
+                        topk_ins<K>(tk[gg][j], out);
                     }
-                    topk_ins<K>(tk[j], out);
                 }
             }
         }
-        // ---- combine + streaming winner (_sweep.h:181-214)
+        // ---- combine + streaming winner of each plane of the group, in order (_sweep.h:181-214)
 #pragma unroll
-        for (int j = 0; j < kVR; ++j) {
-            float s = 0.0f;
+        for (int gg = 0; gg < G; ++gg) {
+            if (gg >= ng)
+                break;
+            const int ii = i + gg;
 #pragma unroll
-            for (int a = 0; a < K; ++a)
-                s = s + tk[j][a];
-            s = s / (float)K;
-            if (st_idx[j] == i - 1)
-                st_snb[j] = s;
-            if (s > st_best[j]) {
-                st_best[j] = s;
-                st_spb[j] = st_prev[j];
-                st_snb[j] = -2.0f;
-                st_idx[j] = i;
+            for (int j = 0; j < kVR; ++j) {
+                float s = 0.0f;
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+                    s = s + tk[gg][j][a];
+                s = s / (float)K;
+                if (st_idx[j] == ii - 1)
+                    st_snb[j] = s;
+                if (s > st_best[j]) {
+                    st_best[j] = s;
+                    st_spb[j] = st_prev[j];
+                    st_snb[j] = -2.0f;
+                    st_idx[j] = ii;
+                }
+                st_prev[j] = s;
             }
-            st_prev[j] = s;
         }
     }
 #pragma unroll
@@ -440,18 +554,19 @@ extern "C" int om_sweep_lp(const om_sweep_desc *d, int32_t fmt, const void *ref1
     if (tiles == 0 || P.nh == 0)
         return OM_OK;
     const bool cd = P.mode == OM_HYP_CONST;
-    const int HR = kTH + 2 * P.r, HCP = (kTW + 2 * P.r) | 1;
-    const size_t lds = (size_t)HR * HCP * (cd ? 3 : 4) * sizeof(float) + 4 * sizeof(float) * (size_t)HR * (kTW + 1);
+    const size_t lds = lp_lds(P.r, cd);
     using KFn = void (*)(const SweepLPK);
-#define OM_LP_ROW(FMT_)                                                                                           \
-    {{sweep_lp_kernel<1, FMT_, false>, sweep_lp_kernel<1, FMT_, true>},                                            \
-     {sweep_lp_kernel<2, FMT_, false>, sweep_lp_kernel<2, FMT_, true>},                                            \
-     {sweep_lp_kernel<3, FMT_, false>, sweep_lp_kernel<3, FMT_, true>},                                            \
-     {sweep_lp_kernel<4, FMT_, false>, sweep_lp_kernel<4, FMT_, true>}}
-    static const KFn table[3][4][2] = {OM_LP_ROW(OM_FMT_F16), OM_LP_ROW(OM_FMT_BF16), OM_LP_ROW(OM_FMT_F32)};
+#define OM_LP_R(K_, FMT_, CD_)                                                                                    \
+    {sweep_lp_kernel<K_, FMT_, CD_, 2>, sweep_lp_kernel<K_, FMT_, CD_, 5>, sweep_lp_kernel<K_, FMT_, CD_, -1>}
+#define OM_LP_K(K_, FMT_) {OM_LP_R(K_, FMT_, false), OM_LP_R(K_, FMT_, true)}
+#define OM_LP_ROW(FMT_) {OM_LP_K(1, FMT_), OM_LP_K(2, FMT_), OM_LP_K(3, FMT_), OM_LP_K(4, FMT_)}
+    static const KFn table[3][4][2][3] = {OM_LP_ROW(OM_FMT_F16), OM_LP_ROW(OM_FMT_BF16), OM_LP_ROW(OM_FMT_F32)};
 #undef OM_LP_ROW
+#undef OM_LP_K
+#undef OM_LP_R
     hipStream_t st = as_stream(stream);
     ProfScope ps("sweep_lp", st);
-    hipLaunchKernelGGL(table[fmt - OM_FMT_F16][k - 1][cd], dim3((unsigned)tiles), dim3(kThreads), lds, st, P);
+    hipLaunchKernelGGL(table[fmt - OM_FMT_F16][k - 1][cd][lp_radius_class(P.r)], dim3((unsigned)tiles),
+                       dim3(kThreads), lds, st, P);
     return launch_check("sweep_lp_kernel");
 }
