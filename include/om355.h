@@ -94,7 +94,10 @@ const char *om_last_error(void);
 /* name (>= 64 bytes), compute units, clock kHz, global memory bytes of the current device */
 int om_device_info(char *name, int32_t name_len, int32_t *cus, int32_t *clock_khz, int64_t *mem_bytes);
 
-/* fused plane sweep: hypotheses [i0, i0+nh) of one view in one launch (+ a merge when chunked) */
+/* fused plane sweep: hypotheses [i0, i0+nh) of one view in one launch (+ a merge when chunked).
+ * d->chunks = 0 chooses the hypothesis chunks for the chip (the count whose rounds of workgroups
+ * sweep the fewest planes) among those whose partials fit d->workspace; om_sweep_workspace_bytes
+ * gives the bytes of that choice unconstrained.  Any chunk count gives the same results. */
 size_t om_sweep_workspace_bytes(const om_sweep_desc *d);
 int om_sweep(const om_sweep_desc *d, void *stream);
 /* The reference's unfused sweep (_dense.ncc_warp per source + _dense.combine, _dense.pyx:13-144,
@@ -110,8 +113,9 @@ int om_sweep_rows(const om_sweep_desc *d, void *stream);
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same descriptor and results as om_sweep; d->workspace is ignored and the hypothesis
  * chunks' partial winner states go to a library-owned static device array of
- * om_sweep_noalloc_max_bytes() bytes (64 MiB) per GPU.  Returns OM_ESHAPE when the chosen chunks
- * need more than that (om_sweep_workspace_bytes(d) > the maximum; pass chunks = 1 to need none).
+ * om_sweep_noalloc_max_bytes() bytes (64 MiB) per GPU; with d->chunks = 0 the count is chosen among
+ * those whose partials fit there.  Returns OM_ESHAPE when an explicit d->chunks needs more (pass
+ * chunks = 0 or 1; one chunk needs none).
  * Every call on one GPU shares that GPU's array: calls that overlap in time on one GPU (other
  * streams or threads) race and must be serialised by the caller. */
 size_t om_sweep_noalloc_max_bytes(void);

@@ -107,8 +107,8 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
 }
 
 // LDS rows of the halo are padded to an odd number of words and the horizontal sums to 65 words
-// per row, so that the horizontal pass (lanes over 8 rows x 8 runs per wave) and the vertical pass
-// (lanes spread over columns) hit few or no bank conflicts.
+// per row, so that neither the horizontal pass (each half-wave over 8 rows x 4 runs) nor the
+// vertical pass (lanes spread over columns) has LDS bank conflicts.
 constexpr int kTWP = kTW + 1;
 
 // The horizontal pass: one task per (halo row, run of kHRun columns), spread over all four waves
@@ -116,7 +116,8 @@ constexpr int kTWP = kTW + 1;
 // step: 32-column runs on one wave issue a quarter fewer instructions yet made the 48-view block's
 // sweep 19 % slower).
 constexpr int kHRun = 8;
-constexpr int kHRuns = kTW / kHRun;
+static_assert(kTW == 8 * kHRun, "a half-wave's 4 runs and their 4 neighbours cover a tile row");
+
 __host__ __device__ constexpr int halo_stride(int r) { return (kTW + 2 * r) | 1; }
 
 // Constant planes are warped in groups of kPlaneGroup per pass over the halo: the plane-independent
@@ -241,9 +242,8 @@ __device__ __forceinline__ void hsum_run(const float *Jr, const float *Ir, int r
 // One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
 // device memory (batched launches).  CD: the hypotheses are constant planes (OM_HYP_CONST), so the
 // depth of a plane is one value for the whole tile and needs no LDS; they are warped in groups of
-// G = kPlaneGroup.  HALF: the horizontal pass's half-wave task order (below).  R: the window radius
-// when it has kernels of its own, else -1 (P.r).
-template <int K, int VR, bool CD, bool HALF, int R, class D>
+// G = kPlaneGroup.  R: the window radius when it has kernels of its own, else -1 (P.r).
+template <int K, int VR, bool CD, int R, class D>
 __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, double *lds_d)
 {
     constexpr int G = plane_group(CD);
@@ -425,26 +425,18 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 if (gg > 0)
                     __syncthreads();  // the previous plane's V pass has read sH
                 const float *sJg = sJ + gg * NHP;
-                // ---- H: horizontal window sums of one 8-column run per task.  A wave takes 8
-                // rows x 8 runs.  HALF: each half-wave takes 8 rows x 4 runs (lane l: row
-                // (l / 4) % 8, run l % 4 + 4 (l / 32)), and with the odd row strides every
-                // half-wave's addresses fall in distinct LDS banks (SQ_LDS_BANK_CONFLICT of a
-                // 2000x1500, S = 8, 128-plane sweep: 0 against 1.1e9 cycles; the sweep 0.4-1.7 %
-                // faster).  The batched kernel keeps 4 rows x 8 runs per half-wave: its
-                // register-bound build is 1.1 % slower with the half-wave order.
-                const int ntask = HALF ? (HR + 7) / 8 * 64 : HR * kHRuns;
+                // ---- H: horizontal window sums of one 8-column run per task.  Each half-wave
+                // takes 8 rows x 4 runs (lane l: row (l / 4) % 8, run l % 4 + 4 (l / 32)), and with
+                // the odd row strides every half-wave's addresses fall in distinct LDS banks
+                // (SQ_LDS_BANK_CONFLICT of a 2000x1500, S = 8, 128-plane sweep: 0 against 1.1e9
+                // cycles with 8 rows x 8 runs per wave; the single-view sweep 0.4-1.7 % faster, the
+                // 48-view block at half scale 368.5 against 368.8 ms).
+                const int ntask = (HR + 7) / 8 * 64;
                 for (int t = tid; t < ntask; t += kThreads) {
-                    int hr, c0;
-                    if (HALF) {
-                        const int l = t & 63;
-                        hr = (t >> 6) * 8 + ((l >> 2) & 7);
-                        c0 = ((l & 3) + 4 * (l >> 5)) * kHRun;
-                        if (hr >= HR)
-                            continue;
-                    } else {
-                        hr = t / kHRuns;
-                        c0 = (t % kHRuns) * kHRun;
-                    }
+                    const int l = t & 63;
+                    const int hr = (t >> 6) * 8 + ((l >> 2) & 7), c0 = ((l & 3) + 4 * (l >> 5)) * kHRun;
+                    if (hr >= HR)
+                        continue;
                     const int o = hr * kTWP + c0;
                     hsum_run<R>(sJg + hr * HCP + c0, sI + hr * HCP + c0, r, sH + o, sH + HR * kTWP + o,
                                 sH + 2 * HR * kTWP + o, sC + o);
@@ -562,7 +554,7 @@ __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_kernel(const Swe
 {
     extern __shared__ double lds_d[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
-    sweep_tile<K, VR, CD, true, R>(P, lb / P.chunks, lb % P.chunks, lds_d);
+    sweep_tile<K, VR, CD, R>(P, lb / P.chunks, lb % P.chunks, lds_d);
 }
 
 // Persistent batch: work items (view, tile, chunk) of the views vidx[0..nv) from one queue; item0[j]
@@ -592,7 +584,7 @@ __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_batch_kernel(con
         }
         const OM_CONST_AS SweepK &P = as_const(views)[as_const(vidx)[lo]];
         const int local = it - as_const(item0)[lo];
-        sweep_tile<K, VR, CD, false, R>(P, local / P.chunks, local % P.chunks, lds_d);
+        sweep_tile<K, VR, CD, R>(P, local / P.chunks, local % P.chunks, lds_d);
     }
 }
 
@@ -852,19 +844,44 @@ static BatchFn batch_kernel(int k, int th, bool cd, int r)
     }
 }
 
-static int choose_chunks(const om_sweep_desc *d, const SweepK &P)
+// Hypothesis chunks of a single-view launch (desc.chunks when set).  All workgroups of a launch
+// do the same work, so tiles x c of them run in ceil(tiles c / slots) rounds, slots = resident
+// workgroups of the chip, each round as long as one chunk of ceil(nh / c) planes plus the tile's
+// set-up and its partials (about one plane): the count with the fewest round-planes, the fewest
+// chunks among equals.  640x480 (300 tiles, 768 slots, 128 planes): 5 chunks, 2 full rounds of 26
+// planes, where 4 chunks left the second of 2 rounds half empty.  Only counts whose partials fit
+// in `limit` bytes are considered (one chunk needs none).
+static size_t part_bytes(int chunks, const SweepK &P);
+
+static int choose_chunks(const om_sweep_desc *d, const SweepK &P, size_t limit = SIZE_MAX)
 {
-    const int64_t tiles = (int64_t)P.tiles_x * P.tiles_y;
+    const int64_t tiles = std::max<int64_t>((int64_t)P.tiles_x * P.tiles_y, 1);
+    const int nh = std::max(d->nh, 1);
     int c = d->chunks;
     if (c <= 0) {
-        int dev = 0, cus = 256;
+        int dev = 0, cus = 256, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             cus = 256;
-        const int64_t want = 4LL * cus;  // ~ resident workgroups of the chip, several rounds
-        c = (int)std::min<int64_t>(std::max<int64_t>(cdiv(want, std::max<int64_t>(tiles, 1)), 1), 64);
+        const bool cd = P.mode == OM_HYP_CONST;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)single_kernel(P.k, P.TH, cd, P.r),
+                                                         kThreads, sweep_lds(P.TH, P.r, cd)) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 3;
+        const int64_t slots = (int64_t)cus * per_cu;
+        int64_t best = INT64_MAX;
+        c = 1;
+        for (int t = 1; t <= std::min(64, nh); ++t) {
+            if (t > 1 && part_bytes(t, P) > limit)
+                break;
+            const int64_t cost = cdiv(tiles * t, slots) * (cdiv(nh, t) + 1);
+            if (cost < best) {
+                best = cost;
+                c = t;
+            }
+        }
     }
-    c = std::max(1, std::min(c, std::max(d->nh, 1)));
+    c = std::max(1, std::min(c, nh));
     return c;
 }
 
@@ -948,9 +965,10 @@ extern "C" int om_sweep(const om_sweep_desc *d, void *stream)
         return rc;
     if (Pl.H == 0 || Pl.W == 0 || Pl.nh == 0)
         return OM_OK;
-    int chunks = choose_chunks(d, Pl);
+    // automatic chunks: the best count whose partials fit the caller's workspace
+    int chunks = choose_chunks(d, Pl, d->workspace ? d->workspace_bytes : 0);
     if (chunks > 1 && (!d->workspace || d->workspace_bytes < part_bytes(chunks, Pl)))
-        chunks = 1;  // no room for partials: one chunk
+        chunks = 1;  // no room for the requested partials: one chunk
     return sweep_launch(Pl, chunks, (float *)d->workspace, as_stream(stream));
 }
 
@@ -962,7 +980,7 @@ extern "C" int om_sweep_noalloc(const om_sweep_desc *d, void *stream)
     int rc = build_params(d, Pl);
     if (rc)
         return rc;
-    const int chunks = choose_chunks(d, Pl);
+    const int chunks = choose_chunks(d, Pl, kNoAllocScratchBytes);  // automatic counts fit the scratch
     const size_t need = part_bytes(chunks, Pl);
     if (need > kNoAllocScratchBytes)
         return fail(OM_ESHAPE,
