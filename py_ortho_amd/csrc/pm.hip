@@ -206,6 +206,19 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 #define OM_PM_RANDOM_ATTR
 #endif
 
+// waves per SIMD of the kernels that evaluate the random tries past the geometric bound (one lane
+// per try, each a different pixel with a random plane: their gathers share nothing, and waves wait
+// 62 % of their time).  5 waves (96 VGPRs with spills) measured slower than 4: the 48-view block's
+// random refinement 526 -> 557 ms a step.
+#ifndef OM_PM_TRY_WAVES
+#define OM_PM_TRY_WAVES OM_PM_WAVES
+#endif
+#if OM_PM_TRY_WAVES > 0
+#define OM_PM_TRY_ATTR __attribute__((amdgpu_waves_per_eu(OM_PM_TRY_WAVES, OM_PM_TRY_WAVES)))
+#else
+#define OM_PM_TRY_ATTR
+#endif
+
 #ifndef OM_PM_GROUP
 #define OM_PM_GROUP 4
 #endif
@@ -1391,7 +1404,7 @@ __global__ void __launch_bounds__(64) pm_try_fallback_kernel(const PMK P, const 
 
 // the listed tries spread evenly over the XCDs, each XCD a contiguous range of the list
 template <int SM>
-__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_try_eval_kernel(const PMK P, const PassK Q, int tr)
+__global__ void __launch_bounds__(kPMThreads) OM_PM_TRY_ATTR pm_try_eval_kernel(const PMK P, const PassK Q, int tr)
 {
     const int64_t n = P.nritems[tr];
     const int64_t used = cdiv(n, (int64_t)blockDim.x), per = cdiv(used, (int64_t)kXCD);
@@ -1493,7 +1506,7 @@ __global__ void __launch_bounds__(64) pm_try_fallback_batch(const PMK *views, co
 // ... and their evaluation: the views' listed tries in blocks of kPMThreads as one sequence (view
 // order), each XCD a contiguous range of it; a block finds its view from the list counts
 template <int SM>
-__global__ void __launch_bounds__(kPMThreads) OM_PM_ATTR pm_try_eval_batch(const PMK *views, int nviews,
+__global__ void __launch_bounds__(kPMThreads) OM_PM_TRY_ATTR pm_try_eval_batch(const PMK *views, int nviews,
                                                                            const PassK Q, int tr)
 {
     __shared__ int s_v, s_lb;
