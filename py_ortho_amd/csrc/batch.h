@@ -47,6 +47,11 @@ struct om_batch {
     unsigned long long *pm_changed = nullptr;  // (pm_n,) in the caller's view order
     std::vector<unsigned long long *> pm_nevals;  // memo path: each view's evaluation count (device)
     std::vector<char> pm_host;    // host copy of the PMK table (memo path: launched view by view)
+    // memo path: the views' propagation chains alternate between side streams, forked from and
+    // joined back to the launch stream around every colour pass
+    int pm_nside = 0;
+    hipStream_t pm_side[4] = {};
+    hipEvent_t pm_fork = nullptr, pm_join[4] = {};
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -67,6 +72,18 @@ struct om_batch {
             (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
+        for (int k = 0; k < 4; ++k) {
+            if (pm_side[k])
+                (void)hipStreamDestroy(pm_side[k]);
+            if (pm_join[k])
+                (void)hipEventDestroy(pm_join[k]);
+            pm_side[k] = nullptr;
+            pm_join[k] = nullptr;
+        }
+        if (pm_fork)
+            (void)hipEventDestroy(pm_fork);
+        pm_fork = nullptr;
+        pm_nside = 0;
         for (void *p : allocs)
             (void)hipFree(p);
         allocs.clear();

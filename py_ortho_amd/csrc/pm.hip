@@ -206,6 +206,13 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 #define OM_PM_RANDOM_ATTR
 #endif
 
+// Streams the batched memoised propagation spreads its views over (1: the launch stream alone).
+// Views alternate between them, so one view's short need / decide kernels and its evaluation's
+// tail overlap another view's evaluation.
+#ifndef OM_PM_SIDE_STREAMS
+#define OM_PM_SIDE_STREAMS 2
+#endif
+
 // waves per SIMD of the kernels that evaluate the random tries past the geometric bound (one lane
 // per try, each a different pixel with a random plane: their gathers share nothing, and waves wait
 // 62 % of their time).  5 waves (96 VGPRs with spills) measured slower than 4: the 48-view block's
@@ -2241,16 +2248,33 @@ int batch_pm_set(om_batch *b, const om_pm_view *views, int nviews, int r, int st
     b->pm_prepare = b->pm_normals;
     for (int v = 0; v < nviews; ++v)
         b->pm_prepare = b->pm_prepare || views[v].depth_from || views[v].score_from;
+    // side streams of the memoised propagation (default priority, non-blocking)
+    const int nside = memo && nviews > 1 ? std::min(OM_PM_SIDE_STREAMS, 4) : 0;
+    if (nside > 1) {
+        if ((rc = hip_check(hipEventCreateWithFlags(&b->pm_fork, hipEventDisableTiming), "hipEventCreate")))
+            return rc;
+        for (int k = 0; k < nside; ++k) {
+            if ((rc = hip_check(hipStreamCreateWithFlags(&b->pm_side[k], hipStreamNonBlocking), "hipStreamCreate")) ||
+                (rc = hip_check(hipEventCreateWithFlags(&b->pm_join[k], hipEventDisableTiming), "hipEventCreate")))
+                return rc;
+        }
+        b->pm_nside = nside;
+    }
     return OM_OK;
 }
 
 // the memoised propagation of one source bucket, view by view: the views of a bucket together hold
 // more source images and depth maps than the last-level cache, one view's fit; the launches are
 // recorded in the batch's graph, so launching per view costs little
-static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, hipStream_t st)
+// the memoised propagation of each view of group g: view i's chain (need, first-iteration sort,
+// eval, decide) on streams[(*next)++ % ns]; a view's chain touches only its own workspace, so
+// chains of different views may run concurrently
+static void pm_group_memo(const om_batch *b, const PMGroup &g, const PassK &Q, const hipStream_t *streams, int ns,
+                          int *next)
 {
     const PMK *host = (const PMK *)b->pm_host.data();
     for (int i = g.first; i < g.first + g.nv; ++i) {
+        const hipStream_t st = streams[(*next)++ % ns];
         const PMK &P = host[i];
         const int64_t ncol = (int64_t)P.H * ((P.W + 1) / 2);
         const unsigned gN = (unsigned)cdiv(ncol, kPassThreads);
@@ -2330,8 +2354,24 @@ static void pm_batch_passes(om_batch *b, hipStream_t st)
                     if (pass > 0)
                         hipLaunchKernelGGL(pm_memo_reset_batch, dim3((unsigned)cdiv(b->pm_n, 64)), dim3(64), 0, st, V,
                                            b->pm_px0, b->pm_n, 0);
-                    for (int gi = 0; gi < b->pm_ngroups; ++gi)
-                        pm_group_memo(b, b->pm_groups[gi], Q, st);
+                    int next = 0;
+                    if (b->pm_nside > 1) {
+                        // fork: the side streams start after everything enqueued so far
+                        // (enqueue errors surface at the launch check of the caller's next kernel)
+                        (void)hipEventRecord(b->pm_fork, st);
+                        for (int k = 0; k < b->pm_nside; ++k)
+                            (void)hipStreamWaitEvent(b->pm_side[k], b->pm_fork, 0);
+                        for (int gi = 0; gi < b->pm_ngroups; ++gi)
+                            pm_group_memo(b, b->pm_groups[gi], Q, b->pm_side, b->pm_nside, &next);
+                        // join: the launch stream continues once every chain has finished
+                        for (int k = 0; k < b->pm_nside; ++k) {
+                            (void)hipEventRecord(b->pm_join[k], b->pm_side[k]);
+                            (void)hipStreamWaitEvent(st, b->pm_join[k], 0);
+                        }
+                    } else {
+                        for (int gi = 0; gi < b->pm_ngroups; ++gi)
+                            pm_group_memo(b, b->pm_groups[gi], Q, &st, 1, &next);
+                    }
                 } else {
                     for (int gi = 0; gi < b->pm_ngroups; ++gi)
                         pm_group_launch(b, b->pm_groups[gi], true, Q, st);
