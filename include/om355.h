@@ -81,7 +81,7 @@ typedef struct om_sweep_desc {
     void *workspace;
     size_t workspace_bytes;
     int32_t chunks;             /* 0 = choose from the image size; 1 = no chunking */
-    int32_t tile_h;             /* 0 = default tile height; else a multiple of 4 */
+    int32_t tile_h;             /* tile rows: 32 (32 columns), 16 or 8 (64 columns); 0 = 32 */
 } om_sweep_desc;
 
 /* library */
@@ -209,7 +209,7 @@ int om_batch_destroy(om_batch *b);
  * every run resets each view's winner state and sweeps its hypotheses [0, nh) (i0 must be 0);
  * depth_out[v] (array may be NULL, entries may be NULL) receives view v's parabola depth;
  * chunks: hypothesis chunks for every view (0 = each desc's chunks, 0 there = auto);
- * tile_h: 8 or 16 for every view (0 = each desc's tile_h, 0 there = 16); views of one tile height
+ * tile_h: 8, 16 or 32 for every view (0 = each desc's tile_h, 0 there = 32); views of one tile height
  * share one persistent launch; grid_blocks: workgroups per launch (0 = all resident). */
 int om_batch_set_sweep(om_batch *b, const om_sweep_desc *views, int32_t nviews, float *const *depth_out,
                        int32_t chunks, int32_t tile_h, int32_t grid_blocks);

@@ -1,5 +1,5 @@
-"""Autotuner of the batched plane sweep: for every image-size class of a block, the tile height
-(8 or 16 rows: LDS footprint and registers against halo overhead) and the number of hypothesis
+"""Autotuner of the batched plane sweep: for every image-size class of a block, the tile shape
+(32 x 32, 64 x 16 or 64 x 8: LDS footprint and registers against halo overhead) and the number of hypothesis
 chunks per view (parallelism against merge traffic), chosen by timing each candidate on the device.
 
 Each candidate runs the sweep of up to `views_per_class` views of the class over the first `planes`
@@ -17,7 +17,7 @@ import torch
 
 from . import batch
 
-CANDIDATES: Tuple[Tuple[int, int], ...] = ((16, 1), (8, 1), (16, 2), (8, 2))
+CANDIDATES: Tuple[Tuple[int, int], ...] = ((32, 1), (16, 1), (8, 1), (32, 2), (16, 2), (8, 2))
 
 
 def _key(cls, s_max, r, k, planes) -> str:

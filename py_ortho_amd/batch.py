@@ -32,7 +32,7 @@ class SweepView:
     hyp: dict
     state: Optional[dense.SweepState] = None
     depth: Optional[torch.Tensor] = None      # receives the parabola depth when given
-    tile_h: int = 0                           # 8 or 16 rows (0 = 16) unless the batch sets one for all
+    tile_h: int = 0                           # 8, 16 or 32 rows (0 = 32) unless the batch sets one for all
     chunks: int = 0                           # hypothesis chunks (0 = auto) unless the batch sets one for all
 
 

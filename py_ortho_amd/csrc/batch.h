@@ -5,6 +5,7 @@
 #include "om_common.h"
 
 // one persistent sweep launch: the views of one tile height and plane kind (constant or per pixel)
+constexpr int kSweepGroups = 6;  // 3 tile heights x 2 plane kinds
 struct SweepGroup {
     int th = 16, first = 0, nv = 0, total = 0, grid = 0;
     bool cd = false;
@@ -22,14 +23,14 @@ struct om_batch {
 
     // plane sweep over all views: one persistent launch per tile height + one post-processing launch
     int sw_n = 0, sw_k = 0, sw_r = 0, sw_ngroups = 0;
-    SweepGroup sw_groups[4];
+    SweepGroup sw_groups[kSweepGroups];
     int64_t sw_px = 0;
     bool sw_merge = false, sw_final = false;
     void *sw_views = nullptr;     // SweepK[sw_n]
     int *sw_vidx = nullptr;       // views ordered by group
     int *sw_item0 = nullptr;      // first work item of each view within its group (order of sw_vidx)
     int64_t *sw_px0 = nullptr;    // first pixel of each view (sw_n + 1)
-    int *sw_counter = nullptr;    // work queue heads, one per group (4)
+    int *sw_counter = nullptr;    // work queue heads, one per group (kSweepGroups)
 
     // PatchMatch over all views: per colour pass, propagation and random refinement once per
     // source bucket, row ranking and stream scan once for all views

@@ -38,14 +38,19 @@
 
 namespace om {
 
-constexpr int kTW = 64;                           // output columns per tile
 constexpr int kThreads = 256;                     // threads per workgroup
-constexpr int kVSeg = kThreads / kTW;             // 4 row segments per column in the vertical pass
 constexpr int kMaxR = 7;                          // largest window radius of the fused kernel
-// Tile heights: 4 * VR output rows, VR = output rows per thread in the vertical pass.  VR = 4
-// (16-row tiles, less halo) is the default; VR = 2 (8-row tiles: less LDS and fewer registers,
-// more workgroups) is the alternative the autotuner can pick for small images.
-constexpr int kVRDefault = 4;
+// Tile shapes (output columns x rows), named by their height: 32 x 32 (the default), 64 x 16 and
+// 64 x 8 (less LDS and fewer registers, more workgroups).  At r = 5 the square tile's 42 x 42 halo
+// is warped in 7 rounds of the workgroup's lanes where 64 x 16's 75 x 26 takes 8, and its
+// horizontal pass fits in three waves: the 48-view block's sweep 1324 against 1431 ms, a 640x480
+// view 1.72 against 1.84 ms; at r = 2 the two shapes tie (1068 ms; 1.43 and 1.44 ms).  The
+// vertical pass gives each thread VR = 4 (or, at 8 rows, 2) output rows of one column: TW columns
+// x 256 / TW row segments.
+constexpr int kTHDefault = 32;
+__host__ __device__ constexpr int tile_w(int th) { return th == 32 ? 32 : 64; }
+__host__ __device__ constexpr int vseg_of(int tw) { return kThreads / tw; }
+__host__ __device__ constexpr int twp_of(int tw) { return tw + 1; }  // row stride of the horizontal sums
 constexpr int kPartPlanes = 6;                    // chunk partials: best, idx, prev-of-best, next-of-best, first, last
 
 struct SrcK {
@@ -106,19 +111,19 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
     }
 }
 
-// LDS rows of the halo are padded to an odd number of words and the horizontal sums to 65 words
-// per row, so that neither the horizontal pass (each half-wave over 8 rows x 4 runs) nor the
-// vertical pass (lanes spread over columns) has LDS bank conflicts.
-constexpr int kTWP = kTW + 1;
+// The horizontal sums' rows are padded to an odd number of words, and so are the 64-wide tiles'
+// halo rows, so that neither the horizontal pass (each half-wave over 8 rows x 4 runs) nor the
+// vertical pass (lanes spread over columns) has LDS bank conflicts.  The 32-wide tiles' halo rows
+// stay unpadded: a pad column adds a warp round at r = 5 (42 x 43 > 7 x 256 pixels), which cost
+// more than the horizontal pass's two-way conflicts save (48-view block: 1397 against 1324 ms).
 
 // The horizontal pass: one task per (halo row, run of kHRun columns), spread over all four waves
 // (the passes are separated by barriers, so a pass on fewer waves lengthens every workgroup's
 // step: 32-column runs on one wave issue a quarter fewer instructions yet made the 48-view block's
 // sweep 19 % slower).
 constexpr int kHRun = 8;
-static_assert(kTW == 8 * kHRun, "a half-wave's 4 runs and their 4 neighbours cover a tile row");
 
-__host__ __device__ constexpr int halo_stride(int r) { return (kTW + 2 * r) | 1; }
+__host__ __device__ constexpr int halo_stride(int r, int tw) { return tw == 64 ? (tw + 2 * r) | 1 : tw + 2 * r; }
 
 // Constant planes are warped in groups of kPlaneGroup per pass over the halo: the plane-independent
 // part of the projection, Rs * ray (_sweep.h:51-53 forms it as its own parenthesised term), is
@@ -131,16 +136,17 @@ constexpr int kPlaneGroup = OM_SWEEP_GROUP;
 __host__ __device__ constexpr int plane_group(bool constd) { return constd ? kPlaneGroup : 1; }
 
 // bytes of the valid-sample counts of the horizontal windows (one byte each, at most 2 r + 1)
-__host__ __device__ constexpr size_t count_bytes(int hr) { return ((size_t)hr * kTWP + 3) & ~(size_t)3; }
+__host__ __device__ constexpr size_t count_bytes(int hr, int tw) { return ((size_t)hr * twp_of(tw) + 3) & ~(size_t)3; }
 
 // LDS bytes of one workgroup.  Per halo pixel its image pixel index, the reference and one warped
 // source value per plane of the group, plus (hypotheses that vary per pixel) the depth of the
 // current plane; three float planes of horizontal sums (J, J^2, J I) and their byte counts.  At
-// r = 5 and 16-row tiles that is 53 170 bytes with constant planes: three workgroups per CU.
+// r = 5 that is 53 170 bytes with constant planes for 64 x 16 tiles and 46 244 for 32 x 32: three
+// workgroups per CU.
 __host__ __device__ constexpr size_t sweep_lds(int th, int r, bool constd)
 {
-    return (size_t)(th + 2 * r) * halo_stride(r) * (constd ? 8 + 4 * plane_group(true) : 20) +
-           (size_t)3 * (th + 2 * r) * kTWP * 4 + count_bytes(th + 2 * r);
+    return (size_t)(th + 2 * r) * halo_stride(r, tile_w(th)) * (constd ? 8 + 4 * plane_group(true) : 20) +
+           (size_t)3 * (th + 2 * r) * twp_of(tile_w(th)) * 4 + count_bytes(th + 2 * r, tile_w(th));
 }
 
 // Window radii with kernels of their own (the window loops unrolled, a horizontal run's samples
@@ -242,16 +248,17 @@ __device__ __forceinline__ void hsum_run(const float *Jr, const float *Ir, int r
 // One (tile, hypothesis chunk) work item of one view; P may live in the kernel arguments or in
 // device memory (batched launches).  CD: the hypotheses are constant planes (OM_HYP_CONST), so the
 // depth of a plane is one value for the whole tile and needs no LDS; they are warped in groups of
-// G = kPlaneGroup.  R: the window radius when it has kernels of its own, else -1 (P.r).
-template <int K, int VR, bool CD, int R, class D>
+// G = kPlaneGroup.  TH: the tile's height (8, 16 or 32 rows), which sets its width.  R: the window
+// radius when it has kernels of its own, else -1 (P.r).
+template <int K, int TH, bool CD, int R, class D>
 __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, double *lds_d)
 {
     constexpr int G = plane_group(CD);
+    constexpr int TW = tile_w(TH), VR = TH / vseg_of(TW), kTWP = twp_of(TW);
     const int r = R >= 0 ? R : P.r;
-    constexpr int TH = kVSeg * VR;
     const int HR = TH + 2 * r;          // halo rows
-    const int HC = kTW + 2 * r;         // halo columns
-    const int HCP = halo_stride(r);     // padded row stride
+    const int HC = TW + 2 * r;          // halo columns
+    const int HCP = halo_stride(r, TW); // row stride
     const int NHP = HR * HCP;           // padded halo; the pad column is outside the image
     double *sD = lds_d;                                          // (HR, HCP) depth of the current hypothesis (!CD)
     int *sP = CD ? (int *)lds_d : (int *)(sD + NHP);             // (HR, HCP) pixel index, -1 outside
@@ -261,7 +268,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     uint8_t *sC = (uint8_t *)(sH + 3 * HR * kTWP);               // (HR, kTWP) valid samples of each window
 
     const int tid = threadIdx.x;
-    const int tx0 = (tile % P.tiles_x) * kTW;
+    const int tx0 = (tile % P.tiles_x) * TW;
     const int ty0 = (tile / P.tiles_x) * TH;
     const int ia = P.i0 + chunk * P.hc;
     const int ib = min(P.i0 + P.nh, ia + P.hc);
@@ -284,7 +291,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     }
 
     // ---- output pixels of this thread in the vertical pass: column vc, rows vy0 .. vy0+VR-1
-    const int vc = tid % kTW, vseg = tid / kTW, vy0 = vseg * VR;
+    const int vc = tid % TW, vseg = tid / TW, vy0 = vseg * VR;
     const int ox = tx0 + vc;
     float mu_[VR], sd_[VR];
     float st_best[VR], st_prev[VR], st_spb[VR], st_snb[VR], st_first[VR];
@@ -426,15 +433,18 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     __syncthreads();  // the previous plane's V pass has read sH
                 const float *sJg = sJ + gg * NHP;
                 // ---- H: horizontal window sums of one 8-column run per task.  Each half-wave
-                // takes 8 rows x 4 runs (lane l: row (l / 4) % 8, run l % 4 + 4 (l / 32)), and with
-                // the odd row strides every half-wave's addresses fall in distinct LDS banks
+                // takes 8 rows x 4 runs (64-wide tiles, lane l: row (l / 4) % 8, run l % 4 +
+                // 4 (l / 32); 32-wide: row (l / 4) % 8 + 8 (l / 32), run l % 4), and with the odd
+                // row strides every half-wave's addresses fall in distinct LDS banks
                 // (SQ_LDS_BANK_CONFLICT of a 2000x1500, S = 8, 128-plane sweep: 0 against 1.1e9
                 // cycles with 8 rows x 8 runs per wave; the single-view sweep 0.4-1.7 % faster, the
                 // 48-view block at half scale 368.5 against 368.8 ms).
-                const int ntask = (HR + 7) / 8 * 64;
+                constexpr int RPW = TW == 64 ? 8 : 16;  // halo rows a wave's 64 tasks cover
+                const int ntask = (HR + RPW - 1) / RPW * 64;
                 for (int t = tid; t < ntask; t += kThreads) {
                     const int l = t & 63;
-                    const int hr = (t >> 6) * 8 + ((l >> 2) & 7), c0 = ((l & 3) + 4 * (l >> 5)) * kHRun;
+                    const int hr = (t >> 6) * RPW + ((l >> 2) & 7) + (TW == 64 ? 0 : 8 * (l >> 5));
+                    const int c0 = ((l & 3) + (TW == 64 ? 4 * (l >> 5) : 0)) * kHRun;
                     if (hr >= HR)
                         continue;
                     const int o = hr * kTWP + c0;
@@ -549,18 +559,18 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
 #endif
 #define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(OM_SWEEP_WAVES, OM_SWEEP_WAVES)))
 
-template <int K, int VR, bool CD, int R>
+template <int K, int TH, bool CD, int R>
 __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_kernel(const SweepK P)
 {
     extern __shared__ double lds_d[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);  // tile-major: an XCD sweeps a band of tiles
-    sweep_tile<K, VR, CD, R>(P, lb / P.chunks, lb % P.chunks, lds_d);
+    sweep_tile<K, TH, CD, R>(P, lb / P.chunks, lb % P.chunks, lds_d);
 }
 
 // Persistent batch: work items (view, tile, chunk) of the views vidx[0..nv) from one queue; item0[j]
 // is the first item of view vidx[j].  Every wave of a workgroup reads the same item index from LDS,
 // so all of them leave the loop together once the queue is empty.
-template <int K, int VR, bool CD, int R>
+template <int K, int TH, bool CD, int R>
 __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_batch_kernel(const SweepK *views, const int *vidx, const int *item0,
                                                                int nv, int total, int *counter)
 {
@@ -584,7 +594,7 @@ __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_batch_kernel(con
         }
         const OM_CONST_AS SweepK &P = as_const(views)[as_const(vidx)[lo]];
         const int local = it - as_const(item0)[lo];
-        sweep_tile<K, VR, CD, R>(P, local / P.chunks, local % P.chunks, lds_d);
+        sweep_tile<K, TH, CD, R>(P, local / P.chunks, local % P.chunks, lds_d);
     }
 }
 
@@ -781,45 +791,49 @@ static int build_params(const om_sweep_desc *d, SweepK &P, bool fused = true)
         if (o.geo[13] != 0.0 || o.geo[14] != 0.0 || o.geo[15] != 0.0)
             P.pinhole = 0;
     }
-    if (d->tile_h != 0 && d->tile_h != 8 && d->tile_h != 16)
-        return fail(OM_EINVAL, "tile height must be 8 or 16 rows (got %d)", d->tile_h);
-    P.TH = d->tile_h ? d->tile_h : kVSeg * kVRDefault;
-    P.tiles_x = (int)cdiv(d->W, kTW);
+    if (d->tile_h != 0 && d->tile_h != 8 && d->tile_h != 16 && d->tile_h != 32)
+        return fail(OM_EINVAL, "tile height must be 8, 16 or 32 rows (got %d)", d->tile_h);
+    P.TH = d->tile_h ? d->tile_h : kTHDefault;
+    P.tiles_x = (int)cdiv(d->W, tile_w(P.TH));
     P.tiles_y = (int)cdiv(d->H, P.TH);
     return OM_OK;
 }
 
-// kernel instances: top-k 1..4, tile height 8 or 16, constant planes or not, radius class
+// kernel instances: top-k 1..4, tile height 8, 16 or 32, constant planes or not, radius class
 using SingleFn = void (*)(const SweepK);
 using BatchFn = void (*)(const SweepK *, const int *, const int *, int, int, int *);
 
-template <int K, int VR, bool CD>
+template <int K, int TH, bool CD>
 static SingleFn single_r(int rc)
 {
-    return rc == 0 ? sweep_kernel<K, VR, CD, 2> : (rc == 1 ? sweep_kernel<K, VR, CD, 5> : sweep_kernel<K, VR, CD, -1>);
+    return rc == 0 ? sweep_kernel<K, TH, CD, 2> : (rc == 1 ? sweep_kernel<K, TH, CD, 5> : sweep_kernel<K, TH, CD, -1>);
 }
 
-template <int K, int VR, bool CD>
+template <int K, int TH, bool CD>
 static BatchFn batch_r(int rc)
 {
-    return rc == 0 ? sweep_batch_kernel<K, VR, CD, 2>
-                   : (rc == 1 ? sweep_batch_kernel<K, VR, CD, 5> : sweep_batch_kernel<K, VR, CD, -1>);
+    return rc == 0 ? sweep_batch_kernel<K, TH, CD, 2>
+                   : (rc == 1 ? sweep_batch_kernel<K, TH, CD, 5> : sweep_batch_kernel<K, TH, CD, -1>);
 }
 
 template <int K>
 static SingleFn single_k(int th, bool cd, int rc)
 {
+    if (th == 32)
+        return cd ? single_r<K, 32, true>(rc) : single_r<K, 32, false>(rc);
     if (th == 16)
-        return cd ? single_r<K, 4, true>(rc) : single_r<K, 4, false>(rc);
-    return cd ? single_r<K, 2, true>(rc) : single_r<K, 2, false>(rc);
+        return cd ? single_r<K, 16, true>(rc) : single_r<K, 16, false>(rc);
+    return cd ? single_r<K, 8, true>(rc) : single_r<K, 8, false>(rc);
 }
 
 template <int K>
 static BatchFn batch_k(int th, bool cd, int rc)
 {
+    if (th == 32)
+        return cd ? batch_r<K, 32, true>(rc) : batch_r<K, 32, false>(rc);
     if (th == 16)
-        return cd ? batch_r<K, 4, true>(rc) : batch_r<K, 4, false>(rc);
-    return cd ? batch_r<K, 2, true>(rc) : batch_r<K, 2, false>(rc);
+        return cd ? batch_r<K, 16, true>(rc) : batch_r<K, 16, false>(rc);
+    return cd ? batch_r<K, 8, true>(rc) : batch_r<K, 8, false>(rc);
 }
 
 static SingleFn single_kernel(int k, int th, bool cd, int r)
@@ -1070,7 +1084,7 @@ static int plan_groups(const SweepK *P, int nviews, int k, int r, int cus, int g
                        std::vector<int> &item0, SweepGroup *groups, int &ng, bool skip_empty)
 {
     ng = 0;
-    for (int th : {16, 8}) {
+    for (int th : {32, 16, 8}) {
         for (int cdi = 1; cdi >= 0; --cdi) {
             const bool cd = cdi != 0;
             SweepGroup &g = groups[ng];
@@ -1109,8 +1123,8 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
 {
     if (nviews < 1 || !views)
         return fail(OM_EINVAL, "batch needs at least one view");
-    if (tile_h != 0 && tile_h != 8 && tile_h != 16)
-        return fail(OM_EINVAL, "tile height must be 8 or 16 rows (got %d)", tile_h);
+    if (tile_h != 0 && tile_h != 8 && tile_h != 16 && tile_h != 32)
+        return fail(OM_EINVAL, "tile height must be 8, 16 or 32 rows (got %d)", tile_h);
     std::vector<SweepK> P(nviews);
     int k = 0, r = -1;
     int64_t tiles = 0;
@@ -1168,7 +1182,7 @@ int batch_sweep_set(om_batch *b, const om_sweep_desc *views, int nviews, float *
     b->sw_vidx = (int *)b->dev_alloc(sizeof(int) * nviews);
     b->sw_item0 = (int *)b->dev_alloc(sizeof(int) * nviews);
     b->sw_px0 = (int64_t *)b->dev_alloc(sizeof(int64_t) * (nviews + 1));
-    b->sw_counter = (int *)b->dev_alloc(sizeof(int) * 4);
+    b->sw_counter = (int *)b->dev_alloc(sizeof(int) * kSweepGroups);
     if (!b->sw_views || !b->sw_vidx || !b->sw_item0 || !b->sw_px0 || !b->sw_counter)
         return fail(OM_ENOMEM, "batch: descriptor tables");
     if ((rc = hip_check(hipMemcpy(b->sw_views, P.data(), sizeof(SweepK) * nviews, hipMemcpyHostToDevice), "upload")) ||
@@ -1191,7 +1205,7 @@ int batch_sweep_enqueue(om_batch *b, hipStream_t st)
     if (b->sw_n == 0)
         return OM_OK;
     ProfScope ps("sweep_batch", st);
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, b->sw_counter, 4);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, b->sw_counter, kSweepGroups);
     hipLaunchKernelGGL(sweep_batch_reset_kernel, dim3((unsigned)cdiv(b->sw_px, 256)), dim3(256), 0, st,
                        (const SweepK *)b->sw_views, b->sw_px0, b->sw_n);
     for (int gi = 0; gi < b->sw_ngroups; ++gi) {
@@ -1258,8 +1272,8 @@ extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float 
 {
     if (nviews < 1 || !views)
         return fail(OM_EINVAL, "om_sweep_multi needs at least one view");
-    if (tile_h != 0 && tile_h != 8 && tile_h != 16)
-        return fail(OM_EINVAL, "tile height must be 8 or 16 rows (got %d)", tile_h);
+    if (tile_h != 0 && tile_h != 8 && tile_h != 16 && tile_h != 32)
+        return fail(OM_EINVAL, "tile height must be 8, 16 or 32 rows (got %d)", tile_h);
     const MultiLayout L = multi_layout(nviews);
     if (!table || table_bytes < L.total || !staging)
         return fail(OM_EINVAL, "om_sweep_multi needs a device table and host staging of %zu bytes", L.total);
@@ -1299,7 +1313,7 @@ extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float 
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    SweepGroup groups[4];
+    SweepGroup groups[kSweepGroups];
     int ng = 0;
     std::vector<int> gv, gi0;
     int rc0 = plan_groups(P, nviews, k, r, cus, 0, gv, gi0, groups, ng, true);
@@ -1315,7 +1329,7 @@ extern "C" int om_sweep_multi(const om_sweep_desc *views, int32_t nviews, float 
     const SweepK *dviews = (const SweepK *)(dtab + L.views);
     int *counter = (int *)(dtab + L.counter);
     ProfScope ps("sweep_multi", st);
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counter, 4);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counter, kSweepGroups);
     for (int gi = 0; gi < ng; ++gi) {
         const SweepGroup &g = groups[gi];
         hipLaunchKernelGGL(batch_kernel(k, g.th, g.cd, r), dim3((unsigned)g.grid), dim3(kThreads), g.lds, st, dviews,

@@ -52,7 +52,7 @@ def _sweep_views(seed=0, r=3, k=2):
     return out
 
 
-@pytest.mark.parametrize("chunks,tile_h", [(0, 0), (1, 16), (3, 8), (2, 16), ("mixed", "mixed")])
+@pytest.mark.parametrize("chunks,tile_h", [(0, 0), (1, 16), (3, 8), (2, 16), (2, 32), ("mixed", "mixed")])
 def test_batch_sweep_equals_oracle_per_view(chunks, tile_h):
     """Every view equals the oracle's sweep + parabola; "mixed" gives each view its own tile height
     and chunk count (two persistent launches, one per tile height)."""
@@ -66,7 +66,7 @@ def test_batch_sweep_equals_oracle_per_view(chunks, tile_h):
         bviews.append(batch.SweepView(_dev(v["ref"]), _dev(v["mu"]), _dev(v["sd"]), [_dev(s) for s in v["srcs"]],
                                       _dev(v["ray"]), v["geo"], v["hyp"],
                                       depth=torch.empty((H, W), dtype=torch.float32, device="cuda"),
-                                      tile_h=(8, 16)[j % 2] if chunks == "mixed" else 0,
+                                      tile_h=(8, 16, 32)[j % 3] if chunks == "mixed" else 0,
                                       chunks=(1, 2, 3)[j % 3] if chunks == "mixed" else 0))
     if chunks == "mixed":
         B.set_sweep(bviews, r, k)

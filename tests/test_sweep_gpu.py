@@ -60,7 +60,7 @@ def test_fused_sweep_matches_oracle(shape):
     from py_ortho_amd import dense
     dsrc = [_dev(s) for s in srcs]
     maps = _dev(np.stack(hyps))
-    for chunks, tile_h in ((1, 0), (0, 0), (3, 0), (2, 8)):  # 0: the default 16-row tiles
+    for chunks, tile_h in ((1, 0), (0, 0), (3, 0), (2, 8), (1, 16)):  # 0: the default 32x32 tiles
         st = dense.sweep(_dev(ref), _dev(mu), _dev(sd), dsrc, _dev(ray), geo, {"mode": "maps", "maps": maps}, r, k,
                          chunks=chunks, tile_h=tile_h)
         torch.cuda.synchronize()
@@ -136,9 +136,10 @@ def test_generators_and_finalize():
     inv = (1.0 / np.linspace(2.0, 30.0, 17)).astype(np.float32)
     hyps = [np.full((H, W), v, np.float32) for v in inv]
     orc = _oracle(ref, mu, sd, srcs, ray, geo, hyps, r, k)
-    st = dense.sweep(*args, {"mode": "const", "inv": _dev(inv)}, r, k)
-    a, ia = _state_np(st)
-    _check_state(orc, a, ia, "const")
+    for tile_h in (0, 16, 8):  # 0: 32x32 tiles
+        st = dense.sweep(*args, {"mode": "const", "inv": _dev(inv)}, r, k, tile_h=tile_h)
+        a, ia = _state_np(st)
+        _check_state(orc, a, ia, ("const", tile_h))
 
 
 def test_depth_helpers_bit_exact():
