@@ -116,6 +116,9 @@ __device__ __forceinline__ void topk_insert(float (&t)[K], float v)
 // vertical pass (lanes spread over columns) has LDS bank conflicts.  The 32-wide tiles' halo rows
 // stay unpadded: a pad column adds a warp round at r = 5 (42 x 43 > 7 x 256 pixels), which cost
 // more than the horizontal pass's two-way conflicts save (48-view block: 1397 against 1324 ms).
+// Pad words between rows that put each half-wave's rows in distinct banks without the extra round
+// (the precision kernel's layout) cut SQ_LDS_BANK_CONFLICT by three quarters here but not the time
+// (r = 5: 1324.7 against 1321.5 ms; r = 2: 1073 against 1067): this kernel waits on its VALU.
 
 // The horizontal pass: one task per (halo row, run of kHRun columns), spread over all four waves
 // (the passes are separated by barriers, so a pass on fewer waves lengthens every workgroup's

@@ -27,10 +27,27 @@
 namespace om {
 namespace lp {
 
-constexpr int kTW = 64;        // output columns per tile
+// 32 x 32 output tiles: at r = 5 the 42 x 42 halo is warped in 7 rounds of the workgroup's lanes
+// and the horizontal sums cover 42 rows per 32 output rows, where 64 x 16 tiles warp 75 x 26 in 8
+// rounds and sum 26 rows per 16 (a 4000x3000 view, 12 sources, 96 planes: 85.0 against 94.4 ms,
+// same bits).  An odd halo stride (43) would add a warp round (89.2 ms); the rows are 32 + 2r words
+// apart instead, with the few pad words of halo32_pad between them.
+constexpr int kTW = 32;        // output columns per tile
 constexpr int kThreads = 256;  // 4 wave64
 constexpr int kVR = 4;         // output rows per thread in the vertical pass
-constexpr int kTH = 16;        // tile height = 4 row segments x kVR
+constexpr int kTH = kThreads / kTW * kVR;  // tile height = 8 row segments x kVR
+__host__ __device__ constexpr int lp_halo_stride(int r) { return kTW + 2 * r; }
+// Pad words before halo row hy (rows hc = 32 + 2r words apart, hc even): chosen from hc mod 8 so that
+// the 8 rows of every aligned group start in 8 distinct classes mod 8.  The horizontal pass gives a
+// half-wave 8 such rows x 4 runs 8 words apart, so its 32 lanes read 32 distinct LDS banks (r = 5:
+// 5 pad words, still 7 warp rounds; SQ_LDS_BANK_CONFLICT down by three quarters and the 4000x3000
+// view's FP16 sweep 85.3 -> 82.9 ms, BF16 84.2 -> 83.2 ms, same bits).
+__host__ __device__ constexpr int halo32_pad(int hy, int hc)
+{
+    return hc % 8 == 0 ? hy : (hc % 8 == 4 ? hy >> 1 : (hy + 4) >> 3);
+}
+__host__ __device__ constexpr int lp_row0(int hy, int r) { return hy * lp_halo_stride(r) + halo32_pad(hy, lp_halo_stride(r)); }
+__host__ __device__ constexpr int lp_halo_words(int hr, int r) { return lp_row0(hr - 1, r) + lp_halo_stride(r); }
 constexpr int kRun = 8;        // columns per horizontal running sum
 constexpr int kMaxR = 7;
 
@@ -115,7 +132,7 @@ __host__ __device__ constexpr size_t lp_count_bytes(int hr) { return ((size_t)hr
 // (per-pixel hypotheses) the plane's depth; three float planes of horizontal sums and their counts
 __host__ __device__ constexpr size_t lp_lds(int r, bool cd)
 {
-    return (size_t)(kTH + 2 * r) * ((kTW + 2 * r) | 1) * 4 * (2 + lp_group(cd) + (cd ? 0 : 1)) +
+    return (size_t)lp_halo_words(kTH + 2 * r, r) * 4 * (2 + lp_group(cd) + (cd ? 0 : 1)) +
            (size_t)3 * (kTH + 2 * r) * (kTW + 1) * 4 + lp_count_bytes(kTH + 2 * r);
 }
 
@@ -206,13 +223,13 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     constexpr int G = lp_group(CD);
     const int r = R >= 0 ? R : P.r;
     const int HR = kTH + 2 * r, HC = kTW + 2 * r;
-    const int HCP = HC | 1;          // padded halo row stride (odd)
+    const int HCP = lp_halo_stride(r);  // halo row stride
     constexpr int kTWP = kTW + 1;    // padded row stride of the horizontal sums
-    const int NH = HR * HCP;
-    float *sD = lds_f;                             // (HR, HCP) depth of the current plane (!CD)
-    int *sP = (int *)(CD ? lds_f : sD + NH);      // (HR, HCP) pixel index, -1 outside
-    float *sI = (float *)(sP + NH);  // (HR, HCP) reference
-    float *sJ = sI + NH;             // (G, HR, HCP) warped source, NaN = invalid
+    const int NH = lp_halo_words(HR, r);  // halo words; pad words are outside the image
+    float *sD = lds_f;                             // (NH) depth of the current plane (!CD)
+    int *sP = (int *)(CD ? lds_f : sD + NH);      // (NH) pixel index, -1 outside
+    float *sI = (float *)(sP + NH);  // (NH) reference
+    float *sJ = sI + NH;             // (G, NH) warped source, NaN = invalid
     float *sH = sJ + G * NH;         // (3, HR, kTWP) horizontal sums of J, J^2, J I
     uint8_t *sC = (uint8_t *)(sH + 3 * HR * kTWP);  // (HR, kTWP) valid samples of each window
 
@@ -223,9 +240,12 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
     const int tid = threadIdx.x;
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
 
-    // the padded halo layout: the pad column (when HC is even) holds index -1 like the outside
+    // the halo: pixel index (-1 outside the image and in pad words) and reference value
     for (int o = tid; o < NH; o += kThreads) {
-        const int hy = o / HCP, hx = o - hy * HCP;
+        int hy = o / HCP;
+        if (o < lp_row0(hy, r))
+            --hy;  // the pads (fewer than HCP words) put o in row hy - 1 or just after it
+        const int hx = o - lp_row0(hy, r);
         const int gy = ty0 - r + hy, gx = tx0 - r + hx;
         const bool in = hx < HC && gy >= 0 && gy < H && gx >= 0 && gx < W;
         const int p = in ? gy * W + gx : -1;
@@ -357,13 +377,12 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                 if (gg > 0)
                     __syncthreads();  // the previous plane's V pass has read sH
                 const float *sJg = sJ + gg * NH;
-                // ---- H: FP32 running sums over 8-column runs; a wave's lanes take 8 rows x 8 runs.
-                // (The FP64 kernel's half-wave order, free of bank conflicts, measured 2-4 % slower
-                // here.)
+                // ---- H: FP32 running sums over 8-column runs; a wave's lanes take 16 rows x 4 runs.
                 for (int t = tid; t < HR * (kTW / kRun); t += kThreads) {
                     const int hr = t / (kTW / kRun), c0 = (t % (kTW / kRun)) * kRun;
                     const int oh = hr * kTWP + c0;
-                    lp_hsum_run<R>(sJg + hr * HCP + c0, sI + hr * HCP + c0, r, sH + oh, sH + HR * kTWP + oh,
+                    const int h0 = lp_row0(hr, r) + c0;
+                    lp_hsum_run<R>(sJg + h0, sI + h0, r, sH + oh, sH + HR * kTWP + oh,
                                    sH + 2 * HR * kTWP + oh, sC + oh);
                 }
                 __syncthreads();

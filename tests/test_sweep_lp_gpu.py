@@ -35,7 +35,10 @@ def _scene():
     return scene.config_c1(device="cuda", scale=0.75)
 
 
-@pytest.mark.parametrize("fmt,r", [("f16", 2), ("bf16", 2), ("f16", 5), ("f32", 2)])
+# f32 storage at the radii without kernels of their own (the generic kernel, and every halo row
+# padding pattern: 32 + 2r = 0, 2, 4 or 6 mod 8)
+@pytest.mark.parametrize("fmt,r", [("f16", 2), ("bf16", 2), ("f16", 5), ("f32", 2), ("f32", 1), ("f32", 3),
+                                   ("f32", 4), ("f32", 6), ("f32", 7)])
 def test_sweep_lp_error_against_fp64(fmt, r):
     from py_ortho_amd import dense, scene
     sc = _scene()
@@ -60,11 +63,13 @@ def test_sweep_lp_error_against_fp64(fmt, r):
     print("lp error", stats)
     # stated error bounds (measured on MI355X: f16 r=2 4.9e-3 / 0.050 / 0.991 / 3.7e-4, f16 r=5
     # 1.2e-3 / 0.011 / 0.997 / 7.1e-5, bf16 r=2 0.038 / 0.32 / 0.893 / 6.6e-3, f32 r=2 6.6e-6 / 6.5e-5 /
-    # 0.9997 / 5.6e-6): median and 99th
+    # 0.9997 / 5.6e-6; f32 at r = 1, 3, 4, 6, 7 at most 1.2e-5 / 1.4e-4 / at least 0.9986 / at most
+    # 9.6e-6): median and 99th
     # percentile of |best score - FP64|, share of identical winning planes, 99th percentile of the
     # relative depth difference, over pixels the reference would keep
     tol = {("f16", 2): (1e-2, 0.1, 0.98, 1e-3), ("f16", 5): (3e-3, 0.03, 0.99, 3e-4),
-           ("bf16", 2): (0.06, 0.5, 0.85, 2e-2), ("f32", 2): (1e-4, 1e-3, 0.998, 1e-4)}[(fmt, r)]
+           ("bf16", 2): (0.06, 0.5, 0.85, 2e-2), ("f32", 2): (1e-4, 1e-3, 0.998, 1e-4)}.get((fmt, r),
+                                                                                       (1e-4, 1e-3, 0.995, 1e-4))
     assert stats[2] < tol[0] and stats[3] < tol[1] and stats[4] > tol[2] and stats[5] < tol[3], stats
 
 
