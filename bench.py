@@ -43,11 +43,11 @@ FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
 HBM_PEAK = 8.0e12
 # HBM bytes of one full-block sweep_batch_kernel launch of the default workload (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply; profiles/block_c3.md)
-C3_SWEEP_TRAFFIC = 3.363e11
+C3_SWEEP_TRAFFIC = 4.566e11
 # share of the SIMD cycles in which that launch issues a VALU instruction (SQ_INSTS_VALU x 4 cycles over
 # 1024 SIMDs x 2.4 GHz x the launch; profiles/block_c3.md): the kernel is issue-bound, and its FLOP
 # fraction is low because a division or square root counts as one FLOP but takes ~11-18 instructions
-C3_SWEEP_VALU_ISSUE = 0.76
+C3_SWEEP_VALU_ISSUE = 0.72
 
 
 def progress(msg: str) -> None:
