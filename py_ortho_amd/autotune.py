@@ -1,6 +1,7 @@
 """Autotuner of the batched plane sweep: for every image-size class of a block, the tile shape
-(32 x 32, 64 x 16 or 64 x 8: LDS footprint and registers against halo overhead) and the number of hypothesis
-chunks per view (parallelism against merge traffic), chosen by timing each candidate on the device.
+(32 x 32, 64 x 16 or 64 x 8: LDS footprint and registers against halo overhead) and the number of
+hypothesis chunks per view (parallelism against merge traffic), chosen by timing each candidate on
+the device.
 
 Each candidate runs the sweep of up to `views_per_class` views of the class over the first `planes`
 hypotheses (the time per hypothesis is constant over a sweep, so a short sweep ranks the candidates
