@@ -141,21 +141,31 @@ __host__ __device__ constexpr int plane_group(bool constd) { return constd ? kPl
 // bytes of the valid-sample counts of the horizontal windows (one byte each, at most 2 r + 1)
 __host__ __device__ constexpr size_t count_bytes(int hr, int tw) { return ((size_t)hr * twp_of(tw) + 3) & ~(size_t)3; }
 
-// LDS bytes of one workgroup.  Per halo pixel its image pixel index, the reference and one warped
-// source value per plane of the group, plus (hypotheses that vary per pixel) the depth of the
-// current plane; three float planes of horizontal sums (J, J^2, J I) and their byte counts.  At
-// r = 5 that is 53 170 bytes with constant planes for 64 x 16 tiles and 46 244 for 32 x 32: three
-// workgroups per CU.
-__host__ __device__ constexpr size_t sweep_lds(int th, int r, bool constd)
-{
-    return (size_t)(th + 2 * r) * halo_stride(r, tile_w(th)) * (constd ? 8 + 4 * plane_group(true) : 20) +
-           (size_t)3 * (th + 2 * r) * twp_of(tile_w(th)) * 4 + count_bytes(th + 2 * r, tile_w(th));
-}
-
+// LDS bytes of one workgroup.  Per halo pixel its image pixel index (except at r = 5), the
+// reference and one warped source value per plane of the group, plus (hypotheses that vary per
+// pixel) the depth of the current plane; three float planes of horizontal sums (J, J^2, J I) and
+// their byte counts.  At r = 5 with constant planes that is 39 188 bytes for 32 x 32 tiles (four
+// workgroups per CU) and 45 372 for 64 x 16 (three).
 // Window radii with kernels of their own (the window loops unrolled, a horizontal run's samples
 // kept in registers): 2 (5x5, densify's default) and 5 (11x11); any other radius <= kMaxR runs the
 // generic kernels (R = -1, radius read from the descriptor).
 __host__ __device__ constexpr int radius_class(int r) { return r == 2 ? 0 : (r == 5 ? 1 : 2); }
+
+// Whether the kernels of a radius class keep each halo pixel's image index in LDS.  The radius-5
+// kernels recompute it instead (for a tile whose halo lies inside the image, base + hp + hy (W - HC):
+// 4 instructions), which brings a 32 x 32 tile's LDS from 46 244 to 39 188 bytes, so four
+// workgroups share a CU: the 48-view block's sweep 1322 -> 1218 ms, same bits (with the index
+// recomputed but three workgroups: 1393 ms).  Keeping instead each lane's horizontal task's
+// reference values in registers (the other 7 KB) spilled hot values: 1430 ms.  Kernels that keep
+// a top-3 or top-4 per pixel need more registers and keep the index in LDS at three workgroups.
+__host__ __device__ constexpr bool idx_lds(int k, int rclass) { return rclass != 1 || k > 2; }
+
+__host__ __device__ constexpr size_t sweep_lds(int th, int r, int k, bool constd)
+{
+    return (size_t)(th + 2 * r) * halo_stride(r, tile_w(th)) *
+               ((idx_lds(k, radius_class(r)) ? 4 : 0) + (constd ? 4 + 4 * plane_group(true) : 16)) +
+           (size_t)3 * (th + 2 * r) * twp_of(tile_w(th)) * 4 + count_bytes(th + 2 * r, tile_w(th));
+}
 
 // Horizontal window sums of one run (_sweep.h:85-114): columns c0 .. c0 + kHRun - 1 of halo row
 // Jr / Ir, sums of V (count), J, J^2 and J I over 2 r + 1 columns, formed in FP64 and stored as
@@ -264,8 +274,9 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const int HCP = halo_stride(r, TW); // row stride
     const int NHP = HR * HCP;           // padded halo; the pad column is outside the image
     double *sD = lds_d;                                          // (HR, HCP) depth of the current hypothesis (!CD)
+    constexpr bool IDX = idx_lds(K, R >= 0 ? radius_class(R) : 2);
     int *sP = CD ? (int *)lds_d : (int *)(sD + NHP);             // (HR, HCP) pixel index, -1 outside
-    float *sI = (float *)(sP + NHP);                             // (HR, HCP) reference
+    float *sI = IDX ? (float *)(sP + NHP) : (float *)sP;       // (HR, HCP) reference
     float *sJ = sI + NHP;                                        // (G, HR, HCP) warped source, NaN = invalid
     float *sH = sJ + G * NHP;                                    // (3, HR, kTWP) horizontal sums of J, J^2, J I
     uint8_t *sC = (uint8_t *)(sH + 3 * HR * kTWP);               // (HR, kTWP) valid samples of each window
@@ -279,6 +290,8 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const size_t HW = (size_t)H * W;
     const bool gen_maps = P.mode == OM_HYP_LINEAR || P.mode == OM_HYP_REFINE;
     const int gx0 = tx0 - r, gy0 = ty0 - r;  // image position of halo pixel (0, 0)
+    const bool inner = HCP == HC && gx0 >= 0 && gy0 >= 0 && gx0 + HC <= W && gy0 + HR <= H;
+    const int pin0 = inner ? gy0 * W + gx0 : 0;
     // image pixel of the halo pixel hp, -1 outside the image (or in the pad column)
     auto halo_px = [&](int hp) -> int {
         const int hy = hp / HCP, hx = hp - hy * HCP;
@@ -289,7 +302,8 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     // ---- the tile + halo: pixel index and reference value
     for (int hp = tid; hp < NHP; hp += kThreads) {
         const int p = halo_px(hp);
-        sP[hp] = p;
+        if (IDX)
+            sP[hp] = p;
         sI[hp] = p >= 0 ? P.ref[p] : 0.f;
     }
 
@@ -342,7 +356,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     dc[gg] = 1.0 / (double)P.ha[i + gg - P.i0];
         } else {
             for (int hp = tid; hp < NHP; hp += kThreads) {
-                const int p = sP[hp];
+                const int p = IDX ? sP[hp] : halo_px(hp);
                 double d = NAN;
                 if (p >= 0) {
                     const float a = gen_maps ? P.ha[p] : 0.f, b = gen_maps ? P.hb[p] : 0.f;
@@ -372,8 +386,17 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
             // works on its own.
             const char *rayx = (const char *)P.ray, *rayy = (const char *)(P.ray + HW),
                        *rayz = (const char *)(P.ray + 2 * HW);
-            int pn = tid < NHP ? sP[tid] : -1;
-            int pn2 = tid + kThreads < NHP ? sP[tid + kThreads] : -1;
+            auto hpx = [&](int hp) -> int {
+                if (IDX)
+                    return sP[hp];
+                if (inner) {  // the whole halo in the image, no pad column: no bounds to check
+                    const int hy = hp / HCP;
+                    return pin0 + hp + hy * (W - HCP);
+                }
+                return halo_px(hp);
+            };
+            int pn = tid < NHP ? hpx(tid) : -1;
+            int pn2 = tid + kThreads < NHP ? hpx(tid + kThreads) : -1;
             uint32_t on = (uint32_t)max(pn, 0) << 2;
             float rxn = *(const float *)(rayx + on), ryn = *(const float *)(rayy + on),
                   rzn = *(const float *)(rayz + on);
@@ -385,7 +408,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 const int p = pn;
                 const double rx = rxn, ry = ryn, rz = rzn;
                 pn = pn2;
-                pn2 = hp + 2 * kThreads < NHP ? sP[hp + 2 * kThreads] : -1;
+                pn2 = hp + 2 * kThreads < NHP ? hpx(hp + 2 * kThreads) : -1;
                 on = (uint32_t)max(pn, 0) << 2;
                 rxn = *(const float *)(rayx + on);
                 ryn = *(const float *)(rayy + on);
@@ -555,12 +578,26 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     }
 }
 
-// Waves per SIMD the fused sweep kernels are compiled for: three 256-thread workgroups share a CU
-// (their LDS fits three times into 160 KB), which needs at most 168 VGPRs per lane.
+// Waves per SIMD the fused sweep kernels are compiled for, that is 256-thread workgroups per CU.
+// The radius-2 and radius-5 kernels of top-1 and top-2: four (at most 128 VGPRs; their LDS fits
+// four times into 160 KB with 32 x 32 tiles).  They then spill 60-308 bytes a lane, mostly the per-pixel winner
+// state that is touched once per plane group, and still gain: a 640x480 view at r = 2 1.43 ->
+// 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic kernels (other radii, up to
+// 7, whose LDS fits three times) and those of top-3 and top-4: three (at most 168 VGPRs).
 #ifndef OM_SWEEP_WAVES
 #define OM_SWEEP_WAVES 3
 #endif
-#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(OM_SWEEP_WAVES, OM_SWEEP_WAVES)))
+#ifndef OM_SWEEP_WAVES_R2
+#define OM_SWEEP_WAVES_R2 4
+#endif
+#ifndef OM_SWEEP_WAVES_R5
+#define OM_SWEEP_WAVES_R5 4
+#endif
+__host__ __device__ constexpr int sweep_waves(int K, int R)
+{
+    return K > 2 ? OM_SWEEP_WAVES : (R == 2 ? OM_SWEEP_WAVES_R2 : (R == 5 ? OM_SWEEP_WAVES_R5 : OM_SWEEP_WAVES));
+}
+#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(sweep_waves(K, R), sweep_waves(K, R))))
 
 template <int K, int TH, bool CD, int R>
 __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_kernel(const SweepK P)
@@ -882,21 +919,33 @@ static int choose_chunks(const om_sweep_desc *d, const SweepK &P, size_t limit =
             cus = 256;
         const bool cd = P.mode == OM_HYP_CONST;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)single_kernel(P.k, P.TH, cd, P.r),
-                                                         kThreads, sweep_lds(P.TH, P.r, cd)) != hipSuccess ||
+                                                         kThreads, sweep_lds(P.TH, P.r, P.k, cd)) != hipSuccess ||
             per_cu < 1)
             per_cu = 3;
         const int64_t slots = (int64_t)cus * per_cu;
-        int64_t best = INT64_MAX;
-        c = 1;
-        for (int t = 1; t <= std::min(64, nh); ++t) {
-            if (t > 1 && part_bytes(t, P) > limit)
-                break;
-            const int64_t cost = cdiv(tiles * t, slots) * (cdiv(nh, t) + 1);
-            if (cost < best) {
-                best = cost;
-                c = t;
+        // Among the counts that give at least two full rounds of resident workgroups (when any
+        // does), the fewest round-planes.  With fewer workgroups than that, CUs run below full
+        // occupancy for much of the launch: a 600x800 view, S = 12, 192 planes, at four workgroups
+        // per CU took 8.5 ms as 2 chunks (one round, the fewest round-planes) and 6.6 ms as 6.
+        for (int pass = 0; pass < 2; ++pass) {
+            int64_t best = INT64_MAX;
+            int pick = 0;
+            for (int t = 1; t <= std::min(64, nh); ++t) {
+                if (t > 1 && part_bytes(t, P) > limit)
+                    break;
+                if (pass == 0 && tiles * t < 2 * slots)
+                    continue;
+                const int64_t cost = cdiv(tiles * t, slots) * (cdiv(nh, t) + 1);
+                if (cost < best) {
+                    best = cost;
+                    pick = t;
+                }
             }
+            c = pick;
+            if (pick > 0)
+                break;
         }
+        c = std::max(c, 1);
     }
     c = std::max(1, std::min(c, nh));
     return c;
@@ -911,7 +960,7 @@ static int sweep_launch(SweepK &Pl, int chunks, float *part, hipStream_t st, boo
     Pl.chunks = (int)cdiv(Pl.nh, Pl.hc);
     Pl.part = part;
     const bool cd = Pl.mode == OM_HYP_CONST;
-    const size_t lds = sweep_lds(Pl.TH, Pl.r, cd);
+    const size_t lds = sweep_lds(Pl.TH, Pl.r, Pl.k, cd);
     const dim3 grid((unsigned)((int64_t)Pl.tiles_x * Pl.tiles_y * Pl.chunks));
     ProfScope ps(prof ? "sweep" : nullptr, st);
     hipLaunchKernelGGL(single_kernel(Pl.k, Pl.TH, cd, Pl.r), grid, dim3(kThreads), lds, st, Pl);
@@ -1108,7 +1157,7 @@ static int plan_groups(const SweepK *P, int nviews, int k, int r, int cus, int g
             if (g.nv == 0 || total == 0)
                 continue;
             g.total = (int)total;
-            g.lds = sweep_lds(th, r, cd);
+            g.lds = sweep_lds(th, r, k, cd);
             int per_cu = 1;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_kernel(k, th, cd, r), kThreads,
                                                              g.lds) != hipSuccess ||
