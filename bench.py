@@ -41,13 +41,15 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpixel-hypotheses/s on 1 GPU; p50/p99 depth-map latency at fixed accuracy"
 FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
 HBM_PEAK = 8.0e12
-# HBM bytes of one full-block sweep_batch_kernel launch of the default workload (FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply; profiles/block_c3.md)
-C3_SWEEP_TRAFFIC = 4.566e11
+# bytes between L2 and memory of one full-block sweep_batch_kernel launch of the default workload
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply;
+# profiles/block_c3.md): mostly registers spilled at four workgroups per CU, which the last-level
+# cache partly absorbs
+C3_SWEEP_TRAFFIC = 1.595e12
 # share of the SIMD cycles in which that launch issues a VALU instruction (SQ_INSTS_VALU x 4 cycles over
 # 1024 SIMDs x 2.4 GHz x the launch; profiles/block_c3.md): the kernel is issue-bound, and its FLOP
 # fraction is low because a division or square root counts as one FLOP but takes ~11-18 instructions
-C3_SWEEP_VALU_ISSUE = 0.72
+C3_SWEEP_VALU_ISSUE = 0.82
 
 
 def progress(msg: str) -> None:
