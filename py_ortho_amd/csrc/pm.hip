@@ -204,8 +204,8 @@ __device__ __forceinline__ double pm_bil(const float *im, int w, double wl, doub
 #ifndef OM_PM_EVAL64_WAVES
 #define OM_PM_EVAL64_WAVES OM_PM_WAVES
 #endif
-#define OM_PM_EVAL_ATTR                                                                                     \
-    __attribute__((amdgpu_waves_per_eu(FMT ? OM_PM_LP_WAVES : OM_PM_EVAL64_WAVES, FMT ? OM_PM_LP_WAVES : OM_PM_EVAL64_WAVES)))
+#define OM_PM_EVAL_WAVES (FMT ? OM_PM_LP_WAVES : OM_PM_EVAL64_WAVES)
+#define OM_PM_EVAL_ATTR __attribute__((amdgpu_waves_per_eu(OM_PM_EVAL_WAVES, OM_PM_EVAL_WAVES)))
 
 // waves per SIMD of the random-refinement kernels (each lane a different pixel, two cost
 // evaluations with the pixel's state live around them)

@@ -44,7 +44,8 @@ constexpr int kMaxR = 7;                          // largest window radius of th
 // 64 x 8 (less LDS and fewer registers, more workgroups).  At r = 5 the square tile's 42 x 42 halo
 // is warped in 7 rounds of the workgroup's lanes where 64 x 16's 75 x 26 takes 8, and its
 // horizontal pass fits in three waves: the 48-view block's sweep 1324 against 1431 ms, a 640x480
-// view 1.72 against 1.84 ms; at r = 2 the two shapes tie (1068 ms; 1.43 and 1.44 ms).  The
+// view 1.72 against 1.84 ms (both at three workgroups per CU); at r = 2 the two shapes tie
+// (1068 ms; 1.43 and 1.44 ms).  The
 // vertical pass gives each thread VR = 4 (or, at 8 rows, 2) output rows of one column: TW columns
 // x 256 / TW row segments.
 constexpr int kTHDefault = 32;
@@ -580,9 +581,9 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
 
 // Waves per SIMD the fused sweep kernels are compiled for, that is 256-thread workgroups per CU.
 // The radius-2 and radius-5 kernels of top-1 and top-2: four (at most 128 VGPRs; their LDS fits
-// four times into 160 KB with 32 x 32 tiles).  They then spill 60-308 bytes a lane, mostly the per-pixel winner
-// state that is touched once per plane group, and still gain: a 640x480 view at r = 2 1.43 ->
-// 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic kernels (other radii, up to
+// four times into 160 KB with 32 x 32 tiles).  They then spill 60-308 bytes a lane, mostly the
+// per-pixel winner state that is touched once per plane group, and still gain: a 640x480 view at
+// r = 2 1.43 -> 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic kernels (other radii, up to
 // 7, whose LDS fits three times) and those of top-3 and top-4: three (at most 168 VGPRs).
 #ifndef OM_SWEEP_WAVES
 #define OM_SWEEP_WAVES 3

@@ -46,7 +46,10 @@ __host__ __device__ constexpr int halo32_pad(int hy, int hc)
 {
     return hc % 8 == 0 ? hy : (hc % 8 == 4 ? hy >> 1 : (hy + 4) >> 3);
 }
-__host__ __device__ constexpr int lp_row0(int hy, int r) { return hy * lp_halo_stride(r) + halo32_pad(hy, lp_halo_stride(r)); }
+__host__ __device__ constexpr int lp_row0(int hy, int r)
+{
+    return hy * lp_halo_stride(r) + halo32_pad(hy, lp_halo_stride(r));
+}
 __host__ __device__ constexpr int lp_halo_words(int hr, int r) { return lp_row0(hr - 1, r) + lp_halo_stride(r); }
 constexpr int kRun = 8;        // columns per horizontal running sum
 constexpr int kMaxR = 7;
