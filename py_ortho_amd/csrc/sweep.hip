@@ -483,11 +483,12 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                 {
                     const float *h1 = sH + vc, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
                     const uint8_t *h0 = sC + vc;
-                    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+                    int c0 = 0;  // valid samples: a small integer, exact either way
+                    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
 #pragma unroll
                     for (int t = 0; t <= 2 * r; ++t) {
                         const int o = (vy0 + t) * kTWP;
-                        c0 = c0 + (double)h0[o];
+                        c0 += h0[o];
                         c1 = c1 + (double)h1[o];
                         c2 = c2 + (double)h2[o];
                         c3 = c3 + (double)h3[o];
@@ -497,8 +498,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                         if (j > 0) {
                             const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
                             double t;
-                            t = c0 + (double)h0[on];
-                            c0 = t - (double)h0[oo];
+                            c0 += (int)h0[on] - (int)h0[oo];
                             t = c1 + (double)h1[on];
                             c1 = t - (double)h1[oo];
                             t = c2 + (double)h2[on];
@@ -507,7 +507,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                             c3 = t - (double)h3[oo];
                         }
                         float out;
-                        const double cov = c0 * kk;
+                        const double cov = (double)c0 * kk;
                         if (cov <= 0.9) {
                             out = -1.0f;
                         } else {

@@ -416,11 +416,12 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                 {
                     const float *h1 = sH + vc, *h2 = h1 + HR * kTWP, *h3 = h2 + HR * kTWP;
                     const uint8_t *h0 = sC + vc;
-                    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+                    int c0 = 0;  // valid samples: a small integer, exact either way
+                    float c1 = 0.f, c2 = 0.f, c3 = 0.f;
 #pragma unroll
                     for (int a = 0; a <= 2 * r; ++a) {
                         const int o = (vy0 + a) * kTWP;
-                        c0 += (float)h0[o];
+                        c0 += h0[o];
                         c1 += h1[o];
                         c2 += h2[o];
                         c3 += h3[o];
@@ -429,14 +430,14 @@ __global__ void __launch_bounds__(kThreads) sweep_lp_kernel(const SweepLPK P)
                     for (int j = 0; j < kVR; ++j) {
                         if (j > 0) {
                             const int on = (vy0 + j + 2 * r) * kTWP, oo = (vy0 + j - 1) * kTWP;
-                            c0 += (float)h0[on] - (float)h0[oo];
+                            c0 += (int)h0[on] - (int)h0[oo];
                             c1 += h1[on] - h1[oo];
                             c2 += h2[on] - h2[oo];
                             c3 += h3[on] - h3[oo];
                         }
                         float out = -1.0f;
-                        if (c0 > 0.9f * win) {  // >= 90 % of the window valid
-                            const float ic = __builtin_amdgcn_rcpf(c0);
+                        if ((float)c0 > 0.9f * win) {  // >= 90 % of the window valid
+                            const float ic = __builtin_amdgcn_rcpf((float)c0);
                             const float mj = c1 * ic, m2 = c2 * ic, mij = c3 * ic;
                             const double vj = (double)m2 - (double)mj * (double)mj;
                             const float sj = vj > 1e-6 ? (float)sqrt(vj) : 1e-3f;
