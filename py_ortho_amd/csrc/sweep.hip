@@ -158,13 +158,14 @@ __host__ __device__ constexpr int radius_class(int r) { return r == 2 ? 0 : (r =
 // workgroups share a CU: the 48-view block's sweep 1322 -> 1218 ms, same bits (with the index
 // recomputed but three workgroups: 1393 ms).  Keeping instead each lane's horizontal task's
 // reference values in registers (the other 7 KB) spilled hot values: 1430 ms.  Kernels that keep
-// a top-3 or top-4 per pixel need more registers and keep the index in LDS at three workgroups.
-__host__ __device__ constexpr bool idx_lds(int k, int rclass) { return rclass != 1 || k > 2; }
+// a top-3 or top-4 per pixel need more registers, and 64-wide tiles at r = 5 need 45 372 bytes
+// even without the indices: both keep the index in LDS at three workgroups.
+__host__ __device__ constexpr bool idx_lds(int k, int rclass, int tw) { return rclass != 1 || k > 2 || tw != 32; }
 
 __host__ __device__ constexpr size_t sweep_lds(int th, int r, int k, bool constd)
 {
     return (size_t)(th + 2 * r) * halo_stride(r, tile_w(th)) *
-               ((idx_lds(k, radius_class(r)) ? 4 : 0) + (constd ? 4 + 4 * plane_group(true) : 16)) +
+               ((idx_lds(k, radius_class(r), tile_w(th)) ? 4 : 0) + (constd ? 4 + 4 * plane_group(true) : 16)) +
            (size_t)3 * (th + 2 * r) * twp_of(tile_w(th)) * 4 + count_bytes(th + 2 * r, tile_w(th));
 }
 
@@ -275,7 +276,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const int HCP = halo_stride(r, TW); // row stride
     const int NHP = HR * HCP;           // padded halo; the pad column is outside the image
     double *sD = lds_d;                                          // (HR, HCP) depth of the current hypothesis (!CD)
-    constexpr bool IDX = idx_lds(K, R >= 0 ? radius_class(R) : 2);
+    constexpr bool IDX = idx_lds(K, R >= 0 ? radius_class(R) : 2, TW);
     int *sP = CD ? (int *)lds_d : (int *)(sD + NHP);             // (HR, HCP) pixel index, -1 outside
     float *sI = IDX ? (float *)(sP + NHP) : (float *)sP;       // (HR, HCP) reference
     float *sJ = sI + NHP;                                        // (G, HR, HCP) warped source, NaN = invalid
@@ -580,11 +581,12 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
 }
 
 // Waves per SIMD the fused sweep kernels are compiled for, that is 256-thread workgroups per CU.
-// The radius-2 and radius-5 kernels of top-1 and top-2: four (at most 128 VGPRs; their LDS fits
-// four times into 160 KB with 32 x 32 tiles).  They then spill 60-308 bytes a lane, mostly the
-// per-pixel winner state that is touched once per plane group, and still gain: a 640x480 view at
-// r = 2 1.43 -> 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic kernels (other radii, up to
-// 7, whose LDS fits three times) and those of top-3 and top-4: three (at most 168 VGPRs).
+// The radius-2 kernels and the radius-5 kernels of 32 x 32 tiles, of top-1 and top-2: four (at
+// most 128 VGPRs; their LDS fits four times into 160 KB).  They then spill 60-308 bytes a lane,
+// mostly the per-pixel winner state that is touched once per plane group, and still gain: a
+// 640x480 view at r = 2 1.43 -> 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic
+// kernels (other radii, up to 7), the radius-5 kernels of 64-wide tiles (whose LDS fits three
+// times) and those of top-3 and top-4: three (at most 168 VGPRs).
 #ifndef OM_SWEEP_WAVES
 #define OM_SWEEP_WAVES 3
 #endif
@@ -594,11 +596,12 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
 #ifndef OM_SWEEP_WAVES_R5
 #define OM_SWEEP_WAVES_R5 4
 #endif
-__host__ __device__ constexpr int sweep_waves(int K, int R)
+__host__ __device__ constexpr int sweep_waves(int K, int TH, int R)
 {
-    return K > 2 ? OM_SWEEP_WAVES : (R == 2 ? OM_SWEEP_WAVES_R2 : (R == 5 ? OM_SWEEP_WAVES_R5 : OM_SWEEP_WAVES));
+    return K > 2 ? OM_SWEEP_WAVES
+                 : (R == 2 ? OM_SWEEP_WAVES_R2 : (R == 5 && TH == 32 ? OM_SWEEP_WAVES_R5 : OM_SWEEP_WAVES));
 }
-#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(sweep_waves(K, R), sweep_waves(K, R))))
+#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(sweep_waves(K, TH, R), sweep_waves(K, TH, R))))
 
 template <int K, int TH, bool CD, int R>
 __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_kernel(const SweepK P)
