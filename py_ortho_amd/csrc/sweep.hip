@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "om_common.h"
 
@@ -911,21 +912,46 @@ static BatchFn batch_kernel(int k, int th, bool cd, int r)
 // in `limit` bytes are considered (one chunk needs none).
 static size_t part_bytes(int chunks, const SweepK &P);
 
+// Resident workgroups per CU of a kernel at an LDS size on the current device, remembered per
+// (device, kernel, LDS) so that a sweep call does not query the runtime each time.  fallback when
+// the query fails.
+static int blocks_per_cu(int dev, const void *fn, size_t lds, int fallback)
+{
+    struct Entry {
+        int dev;
+        const void *fn;
+        size_t lds;
+        int per_cu;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> seen;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (const Entry &e : seen)
+            if (e.dev == dev && e.fn == fn && e.lds == lds)
+                return e.per_cu;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, lds) != hipSuccess || per_cu < 1)
+        return fallback;  // not remembered: a later call asks again
+    std::lock_guard<std::mutex> g(mu);
+    seen.push_back({dev, fn, lds, per_cu});
+    return per_cu;
+}
+
 static int choose_chunks(const om_sweep_desc *d, const SweepK &P, size_t limit = SIZE_MAX)
 {
     const int64_t tiles = std::max<int64_t>((int64_t)P.tiles_x * P.tiles_y, 1);
     const int nh = std::max(d->nh, 1);
     int c = d->chunks;
     if (c <= 0) {
-        int dev = 0, cus = 256, per_cu = 0;
+        int dev = 0, cus = 256;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             cus = 256;
         const bool cd = P.mode == OM_HYP_CONST;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)single_kernel(P.k, P.TH, cd, P.r),
-                                                         kThreads, sweep_lds(P.TH, P.r, P.k, cd)) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 3;
+        const int per_cu = blocks_per_cu(dev, (const void *)single_kernel(P.k, P.TH, cd, P.r),
+                                         sweep_lds(P.TH, P.r, P.k, cd), 3);
         const int64_t slots = (int64_t)cus * per_cu;
         // Among the counts that give at least two full rounds of resident workgroups (when any
         // does), the fewest round-planes.  With fewer workgroups than that, CUs run below full

@@ -51,31 +51,35 @@ class SweepState:
     start at -2, idx at -1."""
 
     def __init__(self, H: int, W: int, device="cuda"):
-        self.best = torch.full((H, W), -2.0, dtype=torch.float32, device=device)
-        self.prev = torch.full((H, W), -2.0, dtype=torch.float32, device=device)
-        self.s_prev_best = torch.full((H, W), -2.0, dtype=torch.float32, device=device)
-        self.s_next_best = torch.full((H, W), -2.0, dtype=torch.float32, device=device)
+        # the four score planes share one allocation, so a fresh or reset state costs two fills
+        self._scores = torch.full((4, H, W), -2.0, dtype=torch.float32, device=device)
+        self.best, self.prev, self.s_prev_best, self.s_next_best = self._scores.unbind(0)
         self.idx = torch.full((H, W), -1, dtype=torch.int32, device=device)
 
     def tensors(self):
         return self.best, self.prev, self.s_prev_best, self.s_next_best, self.idx
 
     def reset(self):
-        for t in (self.best, self.prev, self.s_prev_best, self.s_next_best):
-            t.fill_(-2.0)
+        planes = (self.best, self.prev, self.s_prev_best, self.s_next_best)
+        shared = getattr(self, "_scores", None)
+        if shared is not None and all(t.data_ptr() == shared[k].data_ptr() for k, t in enumerate(planes)):
+            shared.fill_(-2.0)
+        else:
+            for t in planes:
+                t.fill_(-2.0)
         self.idx.fill_(-1)
 
 
 def _src_array(srcs: Sequence[torch.Tensor], geo: np.ndarray, dtype=torch.float32):
     n = len(srcs)
     arr = (OmSrc * max(n, 1))()
+    g = np.ascontiguousarray(geo, dtype=np.float64)
     for s, img in enumerate(srcs):
         if img.dim() != 2:
             raise ValueError("sources must be 2-D images")
         arr[s].img = _need(img, f"srcs[{s}]", dtype)
         arr[s].h, arr[s].w = int(img.shape[0]), int(img.shape[1])
-        for t in range(18):
-            arr[s].geo[t] = float(geo[s, t])
+        ctypes.memmove(ctypes.addressof(arr[s].geo), g[s].ctypes.data, 18 * 8)
     return arr
 
 
