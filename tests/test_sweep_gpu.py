@@ -142,6 +142,25 @@ def test_generators_and_finalize():
         _check_state(orc, a, ia, ("const", tile_h))
 
 
+@pytest.mark.parametrize("k", [1, 3, 4])
+def test_const_planes_r5_topk_variants(k):
+    """Constant planes at r = 5 with top-1 (the four-workgroup kernels, halo indices recomputed)
+    and top-3 / top-4 (halo indices in LDS, three workgroups), 32 x 32 and 64 x 16 tiles, on an
+    image with interior and border tiles, against the oracle."""
+    from py_ortho_amd import dense
+    rng = np.random.default_rng(40 + k)
+    H, W, n, r = 100, 130, 6, 5
+    ref, mu, sd, srcs, ray, geo, _ = cases.sweep_case(rng, H, W, n, r, 1)
+    inv = (1.0 / np.linspace(2.0, 30.0, 13)).astype(np.float32)
+    hyps = [np.full((H, W), v, np.float32) for v in inv]
+    orc = _oracle(ref, mu, sd, srcs, ray, geo, hyps, r, k)
+    args = (_dev(ref), _dev(mu), _dev(sd), [_dev(s) for s in srcs], _dev(ray), geo)
+    for tile_h, chunks in ((0, 0), (16, 1), (0, 3)):
+        st = dense.sweep(*args, {"mode": "const", "inv": _dev(inv)}, r, k, tile_h=tile_h, chunks=chunks)
+        a, ia = _state_np(st)
+        _check_state(orc, a, ia, ("const r5", k, tile_h, chunks))
+
+
 def test_depth_helpers_bit_exact():
     from py_ortho_amd import dense
     rng = np.random.default_rng(14)
