@@ -170,20 +170,40 @@ __host__ __device__ constexpr size_t sweep_lds(int th, int r, int k, bool constd
            (size_t)3 * (th + 2 * r) * twp_of(tile_w(th)) * 4 + count_bytes(th + 2 * r, tile_w(th));
 }
 
+// Waves per SIMD the fused sweep kernels are compiled for, that is 256-thread workgroups per CU.
+// The radius-2 kernels and the radius-5 kernels of 32 x 32 tiles, of top-1 and top-2: four (at
+// most 128 VGPRs; their LDS fits four times into 160 KB).  They then spill 32-124 bytes a lane,
+// mostly the per-pixel winner state that is touched once per plane group, and still gain: a
+// 640x480 view at r = 2 1.43 -> 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic
+// kernels (other radii, up to 7), the radius-5 kernels of 64-wide tiles (whose LDS fits three
+// times) and those of top-3 and top-4: three (at most 168 VGPRs).
+#ifndef OM_SWEEP_WAVES
+#define OM_SWEEP_WAVES 3
+#endif
+#ifndef OM_SWEEP_WAVES_R2
+#define OM_SWEEP_WAVES_R2 4
+#endif
+#ifndef OM_SWEEP_WAVES_R5
+#define OM_SWEEP_WAVES_R5 4
+#endif
+__host__ __device__ constexpr int sweep_waves(int K, int TH, int R)
+{
+    return K > 2 ? OM_SWEEP_WAVES
+                 : (R == 2 ? OM_SWEEP_WAVES_R2 : (R == 5 && TH == 32 ? OM_SWEEP_WAVES_R5 : OM_SWEEP_WAVES));
+}
+#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(sweep_waves(K, TH, R), sweep_waves(K, TH, R))))
+
 // Horizontal window sums of one run (_sweep.h:85-114): columns c0 .. c0 + kHRun - 1 of halo row
 // Jr / Ir, sums of V (count), J, J^2 and J I over 2 r + 1 columns, formed in FP64 and stored as
 // float exactly where the reference rounds them.  Each column adds the entering sample, then
-// subtracts the leaving one (the reference's order).  With a compile-time radius each sample's
-// validity and products are formed once and kept in registers until it leaves the window.
-#ifndef OM_SWEEP_HCACHE_MAXR
-#define OM_SWEEP_HCACHE_MAXR 7
-#endif
-
-template <int R>
+// subtracts the leaving one (the reference's order).  CACHE (a compile-time radius R >= 0): each
+// sample's validity and products are formed once and kept in registers until it leaves the
+// window; else the leaving sample is read again.
+template <int R, bool CACHE>
 __device__ __forceinline__ void hsum_run(const float *Jr, const float *Ir, int r, float *h1, float *h2, float *h3,
                                          uint8_t *hc)
 {
-    if constexpr (R >= 0 && R <= OM_SWEEP_HCACHE_MAXR) {
+    if constexpr (R >= 0 && CACHE) {
         constexpr int NE = kHRun + 2 * R;
         float ej[NE], ejj[NE], eji[NE];
         int eok[NE];
@@ -278,6 +298,10 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
     const int NHP = HR * HCP;           // padded halo; the pad column is outside the image
     double *sD = lds_d;                                          // (HR, HCP) depth of the current hypothesis (!CD)
     constexpr bool IDX = idx_lds(K, R >= 0 ? radius_class(R) : 2, TW);
+    // The horizontal pass keeps its samples in registers except in the four-wave radius-5 kernels:
+    // at 128 VGPRs the cache made them spill more than it saved (the 48-view block's sweep 1205 ->
+    // 1182 ms without it, spills 312 -> 112 bytes; at r = 2 the cache still pays, 926 -> 899 ms).
+    constexpr bool HCACHE = R >= 0 && (R == 2 || sweep_waves(K, TH, R) < 4);
     int *sP = CD ? (int *)lds_d : (int *)(sD + NHP);             // (HR, HCP) pixel index, -1 outside
     float *sI = IDX ? (float *)(sP + NHP) : (float *)sP;       // (HR, HCP) reference
     float *sJ = sI + NHP;                                        // (G, HR, HCP) warped source, NaN = invalid
@@ -477,7 +501,7 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
                     if (hr >= HR)
                         continue;
                     const int o = hr * kTWP + c0;
-                    hsum_run<R>(sJg + hr * HCP + c0, sI + hr * HCP + c0, r, sH + o, sH + HR * kTWP + o,
+                    hsum_run<R, HCACHE>(sJg + hr * HCP + c0, sI + hr * HCP + c0, r, sH + o, sH + HR * kTWP + o,
                                 sH + 2 * HR * kTWP + o, sC + o);
                 }
                 __syncthreads();
@@ -580,29 +604,6 @@ __device__ __forceinline__ void sweep_tile(const D &P, int tile, int chunk, doub
         }
     }
 }
-
-// Waves per SIMD the fused sweep kernels are compiled for, that is 256-thread workgroups per CU.
-// The radius-2 kernels and the radius-5 kernels of 32 x 32 tiles, of top-1 and top-2: four (at
-// most 128 VGPRs; their LDS fits four times into 160 KB).  They then spill 60-308 bytes a lane,
-// mostly the per-pixel winner state that is touched once per plane group, and still gain: a
-// 640x480 view at r = 2 1.43 -> 1.23 ms, the 48-view block at r = 2 1070 -> 924 ms.  The generic
-// kernels (other radii, up to 7), the radius-5 kernels of 64-wide tiles (whose LDS fits three
-// times) and those of top-3 and top-4: three (at most 168 VGPRs).
-#ifndef OM_SWEEP_WAVES
-#define OM_SWEEP_WAVES 3
-#endif
-#ifndef OM_SWEEP_WAVES_R2
-#define OM_SWEEP_WAVES_R2 4
-#endif
-#ifndef OM_SWEEP_WAVES_R5
-#define OM_SWEEP_WAVES_R5 4
-#endif
-__host__ __device__ constexpr int sweep_waves(int K, int TH, int R)
-{
-    return K > 2 ? OM_SWEEP_WAVES
-                 : (R == 2 ? OM_SWEEP_WAVES_R2 : (R == 5 && TH == 32 ? OM_SWEEP_WAVES_R5 : OM_SWEEP_WAVES));
-}
-#define OM_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(sweep_waves(K, TH, R), sweep_waves(K, TH, R))))
 
 template <int K, int TH, bool CD, int R>
 __global__ void __launch_bounds__(kThreads) OM_SWEEP_ATTR sweep_kernel(const SweepK P)
