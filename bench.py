@@ -43,13 +43,13 @@ FP64_PEAK = 78.6e12       # MI355X published FP64 (vector and matrix) TFLOP/s
 HBM_PEAK = 8.0e12
 # bytes between L2 and memory of one full-block sweep_batch_kernel launch of the default workload
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply;
-# profiles/block_c3.md): mostly registers spilled at four workgroups per CU, which the last-level
-# cache partly absorbs
-C3_SWEEP_TRAFFIC = 1.899e12
+# profiles/block_c3.md): taps fetched again through the last-level cache and registers spilled at
+# four workgroups per CU
+C3_SWEEP_TRAFFIC = 6.826e11
 # share of the SIMD cycles in which that launch issues a VALU instruction (SQ_INSTS_VALU x 4 cycles over
 # 1024 SIMDs x 2.4 GHz x the launch; profiles/block_c3.md): the kernel is issue-bound, and its FLOP
 # fraction is low because a division or square root counts as one FLOP but takes ~11-18 instructions
-C3_SWEEP_VALU_ISSUE = 0.81
+C3_SWEEP_VALU_ISSUE = 0.84
 
 
 def progress(msg: str) -> None:
