@@ -6,9 +6,10 @@ the device.
 Each candidate runs the sweep of up to `views_per_class` views of the class over the first `planes`
 hypotheses; the fastest per pixel-hypothesis wins.  Each chunk's set-up is spread over its planes,
 so the tuning sweep must be long enough for the chunk counts it compares: over 16 planes 600x800
-views ranked 2 chunks first, over 48 planes 6 chunks, 11 % faster per pixel-hypothesis (the same
-holds for the full 192).  Results can be kept in a JSON file keyed by class, sources, window and
-plane count.
+views ranked 2 chunks first, over 48 planes 12 and over 96 planes 16 (4 % faster per
+pixel-hypothesis than 6; 900x1200 views 12 chunks, 1200x1600 and 1800x2400 8, the last two within
+0.5 % of each other from 4 to 16).  Results can be kept in a JSON file keyed by class, sources,
+window and plane count.
 """
 from __future__ import annotations
 
@@ -20,7 +21,8 @@ import torch
 
 from . import batch
 
-CANDIDATES: Tuple[Tuple[int, int], ...] = ((32, 1), (32, 2), (32, 4), (32, 6), (16, 1), (16, 2), (8, 1), (8, 2))
+CANDIDATES: Tuple[Tuple[int, int], ...] = ((32, 1), (32, 2), (32, 4), (32, 6), (32, 8), (32, 12), (32, 16),
+                                           (16, 1), (16, 2), (8, 1))
 
 
 def _key(cls, s_max, r, k, planes) -> str:
@@ -40,7 +42,7 @@ def _time_batch(B: batch.Batch, reps: int) -> float:
     return best
 
 
-def tune_block(blk, candidates: Sequence[Tuple[int, int]] = CANDIDATES, planes: int = 48, views_per_class: int = 2,
+def tune_block(blk, candidates: Sequence[Tuple[int, int]] = CANDIDATES, planes: int = 96, views_per_class: int = 2,
                reps: int = 2, cache_path: Optional[str] = None):
     """-> ({(H, W): (tile_h, chunks)}, {(H, W): {(tile_h, chunks): ns per pixel-hypothesis}})."""
     o = blk.opt
