@@ -45,11 +45,11 @@ HBM_PEAK = 8.0e12
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated against a 256 MiB torch multiply;
 # profiles/block_c3.md): taps fetched again through the last-level cache and registers spilled at
 # four workgroups per CU
-C3_SWEEP_TRAFFIC = 6.826e11
+C3_SWEEP_TRAFFIC = 5.069e11
 # share of the SIMD cycles in which that launch issues a VALU instruction (SQ_INSTS_VALU x 4 cycles over
 # 1024 SIMDs x 2.4 GHz x the launch; profiles/block_c3.md): the kernel is issue-bound, and its FLOP
 # fraction is low because a division or square root counts as one FLOP but takes ~11-18 instructions
-C3_SWEEP_VALU_ISSUE = 0.84
+C3_SWEEP_VALU_ISSUE = 0.85
 
 
 def progress(msg: str) -> None:
